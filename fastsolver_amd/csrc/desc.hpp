@@ -1,0 +1,83 @@
+// Batch descriptors shared by the host schedule and the device kernels.  Every
+// batched launch reads an array of descriptors; each descriptor owns a
+// contiguous range of flattened workgroup tiles starting at tile_begin.
+#pragma once
+#include <stdint.h>
+
+namespace hodlr {
+
+// Cout = alpha * op(A) * B + beta * Cin   (op(A) = A or A^T), C is m x n.
+// With nsplit > 1 the K range is cut into chunks of kchunk; chunk s writes its
+// raw product alpha*op(A)*B to Cout + s*split_stride (leading dimension
+// ldcout) and a SumDesc pass adds the chunks in a fixed order.
+struct GemmDesc {
+  const void* A;
+  const void* B;
+  const void* Cin;
+  void* Cout;
+  int64_t lda, ldb, ldcin, ldcout;
+  int32_t m, n, k, kchunk;
+  int32_t tiles_m, tiles_n, nsplit, pad0;
+  int64_t split_stride;
+  double alpha, beta;
+  int64_t tile_begin;
+};
+
+// out = sum_{s < nsplit} P[s*stride + i + j*m]   (fixed order), m x n
+struct SumDesc {
+  const void* P;
+  int64_t stride;
+  void* out;
+  int64_t ldo;
+  int32_t m, n, nsplit, pad0;
+  int64_t tile_begin;
+};
+
+// Batched LU with partial pivoting of square blocks (getrf semantics).
+struct LuDesc {
+  void* A;            // m x m, column-major, leading dimension lda
+  int64_t lda;
+  int32_t m, pad0;
+  int32_t* ipiv;      // m entries, 0-based row swapped with row i at step i
+  int32_t* perm;      // m entries, row i of P*A is row perm[i] of A
+  int32_t* info;      // first zero pivot (1-based), 0 if none
+  int64_t tile_begin; // used by the row-block launch
+};
+
+// X_out = U^-1 L^-1 P X_in for one LU-factored block and ncols right-hand sides.
+struct TrsmDesc {
+  const void* LU;
+  int64_t ldlu;
+  const int32_t* perm;  // may be null: no row interchanges
+  const void* Xin;
+  int64_t ldin;
+  void* Xout;
+  int64_t ldout;
+  int32_t m, ncols;
+  int64_t tile_begin;
+};
+
+// dst = alpha * src + beta * dst + diag * I (rows x cols), types given by the
+// launch.  beta = 0 never reads dst.
+struct CopyDesc {
+  const void* src;
+  int64_t lds;
+  void* dst;
+  int64_t ldd;
+  int32_t rows, cols;
+  double alpha, beta, diag;
+  int64_t tile_begin;
+};
+
+// Sum of squares of one column segment: partial[chunk] for rows
+// [chunk*kNormChunk, ...) of x (n rows).
+struct NormDesc {
+  const double* x;
+  int64_t n;
+  double* partial;     // nchunks entries
+  double* out;         // out = sqrt(sum of partials), written by the finish pass
+  int32_t nchunks, pad0;
+  int64_t tile_begin;
+};
+
+}  // namespace hodlr

@@ -1,0 +1,1187 @@
+// Plan, breadth-first level schedule and C-ABI of the HODLR solver.
+//
+// The reference launches one Legion task per leaf, per reduce, per node solve
+// and per broadcast, walking its breadth-first node list backwards
+// (src/solver/fast_solver.cc:136-255).  Here every phase of every node of the
+// same height, across all systems of the plan, is one batched kernel launch
+// over a descriptor array built once at plan creation:
+//
+//   leaf      K copy -> LU (blocked, 64-wide panels: panel / row-block / update) -> perm
+//             -> TRSM of [b | U] (solver_tasks.cc:424-438)
+//   height h  reduce   W_i = V_i^T X_i[:, 0:c+R_l+r]  (split-K + fixed-order sum)
+//             node     VTd1 -= VTu1 VTd0;  S = I - VTu1 VTu0;  LU(S);  eta0 = S^-1 VTd1;
+//                      eta1 = VTd0 - VTu0 eta0          (solver_tasks.cc:216-302)
+//             bcast    X_i[:, 0:c+R_l] -= X_i[:, ucols] eta_i  (gemm.cc:417-478)
+//
+// Heights replace the reference's reverse BFS order: all children of a node of
+// height h have height < h, so one launch per phase and height respects every
+// dependence Legion derived from region privileges.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fastsolver_amd.h"
+#include "desc.hpp"
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace hodlr {
+namespace {
+
+constexpr int kMaxBlock = 1024;   // largest leaf / rank the LU panel kernel holds in registers
+
+enum Cat {
+  CAT_LEAF_COPY = 0, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, CAT_LEAF_TRSM,
+  CAT_REDUCE, CAT_REDUCE_SUM, CAT_NODE_GEMM, CAT_NODE_LU, CAT_NODE_TRSM, CAT_BCAST,
+  CAT_COPY, CAT_MATVEC, CAT_NORM, CAT_OTHER, CAT_N
+};
+const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_update", "lu_misc", "leaf_trsm",
+                                "reduce_gemm", "reduce_sum", "node_gemm", "node_lu", "node_trsm", "bcast_gemm",
+                                "copy", "matvec", "norm", "other"};
+static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
+
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_ROW, K_LU_PERM, K_TRSM, K_NORM, K_NORM_FIN };
+
+struct Launch {
+  Kind kind;
+  Cat cat;
+  DT t[4];
+  bool trans = false;
+  int role = 0;
+  size_t off = 0;        // byte offset of the first descriptor in the blob
+  int count = 0;
+  int64_t tiles = 0;
+  int k0 = 0, rows_max = 0, mmax = 0;
+  double flops = 0, bytes = 0;   // algorithmic work of the launch (profiling)
+};
+
+struct Proc {
+  std::vector<Launch> l;
+  void append(const Proc& o) { l.insert(l.end(), o.l.begin(), o.l.end()); }
+};
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t esize(DT t) { return t == F64 ? 8 : 4; }
+
+// Device buffer with typed offsets.
+struct Buf {
+  char* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct SysState {
+  hodlr_system_t s;
+  Tree tree;
+  std::vector<int32_t> ranks;
+  int64_t ldw = 0;                 // leading dimension of the workspaces (= N)
+  size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0;
+  std::vector<size_t> lu_off;      // per leaf (elements)
+  std::vector<size_t> z_off;       // per leaf (doubles)
+  std::vector<size_t> vtd, vtu, wmv;   // per node (doubles): reduce outputs
+  std::vector<size_t> sbuf, eta, spiv, sperm;  // per interior node
+  std::vector<int> sinfo;          // per interior node: index into node info array
+  std::vector<size_t> kofs;        // per leaf (elements of K)
+  size_t bnorm = 0, rnorm = 0;     // doubles, c each
+};
+
+}  // namespace
+}  // namespace hodlr
+
+using namespace hodlr;
+
+struct hodlr_plan_s {
+  int precision = HODLR_PREC_FP64;
+  DT wt = F64;                     // workspace element type
+  hipStream_t stream = nullptr;
+  std::vector<SysState> sys;
+  int height = 0, nleaves = 0, nnodes = 0;
+  // device memory
+  Buf ws;        // all workspaces (one allocation)
+  Buf descs;     // all descriptors
+  Buf scratch;   // split-K partials and norm partials
+  size_t scratch_doubles = 0;
+  int32_t* leaf_info = nullptr;    // into ws
+  int32_t* node_info = nullptr;
+  // procedures
+  Proc leaf, extract, residual, update;
+  std::vector<Proc> reduce, node, bcast;          // index = height
+  std::vector<Proc> reduce_s, node_s, bcast_s;    // solve-only (c columns)
+  Proc solve_only_leaf;
+  Proc factor;                                    // everything of the fused pass
+  // hip graph
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  // profiling
+  bool profile = false;
+  double prof_ms[CAT_N] = {0};
+  int32_t prof_n[CAT_N] = {0};
+  double prof_flops[CAT_N] = {0}, prof_bytes[CAT_N] = {0};
+  std::vector<hipEvent_t> ev;
+  // info
+  hodlr_plan_info_t info{};
+  bool factored = false;
+};
+
+namespace {
+
+// ------------------------------------------------------------ descriptor blob
+struct Blob {
+  std::vector<char> data;
+  template <class D> size_t add(const std::vector<D>& v) {
+    size_t off = (data.size() + 255) & ~size_t(255);
+    data.resize(off + v.size() * sizeof(D));
+    memcpy(data.data() + off, v.data(), v.size() * sizeof(D));
+    return off;
+  }
+};
+
+struct Builder {
+  hodlr_plan_s* P;
+  Blob blob;
+  size_t scratch_need = 0;   // doubles
+
+  char* wsp(size_t off) { return P->ws.p + off; }
+
+  // ---- GEMM batches
+  struct G {
+    const void* A; int64_t lda; const void* B; int64_t ldb; const void* Cin; int64_t ldcin; void* Cout;
+    int64_t ldcout; int m, n, k; double alpha, beta;
+  };
+  // split: desired minimum number of workgroups; 0 = never split
+  void gemm(Proc& pr, Cat cat, DT ta, DT tb, DT tc, DT mt, bool trans, int role, const std::vector<G>& gs,
+            int64_t min_wg = 0) {
+    std::vector<GemmDesc> d;
+    std::vector<SumDesc> sums;
+    int64_t base_tiles = 0;
+    for (const G& g : gs)
+      if (g.m > 0 && g.n > 0) base_tiles += cdiv(g.m, kGemmBM) * cdiv(g.n, kGemmBN);
+    int64_t tiles = 0;
+    size_t part = 0;  // doubles of split-K partials
+    double flops = 0, bytes = 0;
+    for (const G& g : gs) {
+      if (g.m <= 0 || g.n <= 0) continue;
+      GemmDesc x{};
+      x.A = g.A; x.B = g.B; x.Cin = g.Cin; x.Cout = g.Cout;
+      x.lda = g.lda; x.ldb = g.ldb; x.ldcin = g.ldcin; x.ldcout = g.ldcout;
+      x.m = g.m; x.n = g.n; x.k = g.k;
+      x.tiles_m = int(cdiv(g.m, kGemmBM));
+      x.tiles_n = int(cdiv(g.n, kGemmBN));
+      x.alpha = g.alpha; x.beta = g.beta;
+      int nsplit = 1;
+      if (min_wg > 0 && base_tiles < min_wg && g.k >= 1024 && g.beta == 0.0 && tc == F64) {
+        int64_t want = cdiv(min_wg, base_tiles);
+        nsplit = int(std::min<int64_t>(want, g.k / 512));
+        nsplit = std::max(1, nsplit);
+      }
+      int kchunk = int(cdiv(cdiv(std::max(g.k, 1), nsplit), kGemmBK) * kGemmBK);
+      nsplit = int(cdiv(std::max(g.k, 1), kchunk));
+      x.kchunk = kchunk;
+      x.nsplit = nsplit;
+      if (nsplit > 1) {
+        // partials go to scratch (doubles: every split-K caller outputs FP64)
+        x.Cout = (void*)(uintptr_t)(part * 8);   // patched to the scratch base below
+        x.ldcout = g.m;
+        x.split_stride = int64_t(g.m) * g.n;
+        SumDesc sd{};
+        sd.P = (const void*)(uintptr_t)(part * 8);
+        sd.stride = x.split_stride;
+        sd.out = g.Cout;
+        sd.ldo = g.ldcout;
+        sd.m = g.m; sd.n = g.n; sd.nsplit = nsplit;
+        sums.push_back(sd);
+        part += size_t(nsplit) * g.m * g.n;
+      }
+      x.tile_begin = tiles;
+      tiles += int64_t(x.tiles_m) * x.tiles_n * nsplit;
+      d.push_back(x);
+      flops += 2.0 * g.m * g.n * g.k;
+      bytes += double(g.m) * g.k * esize(ta) + double(g.k) * g.n * esize(tb) +
+               double(g.m) * g.n * esize(tc) * (g.beta != 0.0 ? 2 : 1);
+    }
+    if (d.empty()) return;
+    if (part) {
+      scratch_need = std::max(scratch_need, part);
+      for (auto& x : d)
+        if (x.nsplit > 1) x.Cout = (void*)(P->scratch.p + (uintptr_t)x.Cout);
+      for (auto& s : sums) s.P = (const void*)(P->scratch.p + (uintptr_t)s.P);
+    }
+    Launch L;
+    L.kind = K_GEMM; L.cat = cat;
+    L.t[0] = ta; L.t[1] = tb; L.t[2] = tc; L.t[3] = mt;
+    L.trans = trans; L.role = role;
+    L.off = blob.add(d); L.count = int(d.size()); L.tiles = tiles;
+    L.flops = flops; L.bytes = bytes;
+    pr.l.push_back(L);
+    if (!sums.empty()) {
+      int64_t st = 0;
+      for (auto& s : sums) { s.tile_begin = st; st += cdiv(int64_t(s.m) * s.n, kSumTile); }
+      Launch S;
+      S.kind = K_SUM; S.cat = cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat;
+      S.off = blob.add(sums); S.count = int(sums.size()); S.tiles = st;
+      pr.l.push_back(S);
+    }
+  }
+
+  void copy(Proc& pr, Cat cat, DT ts, DT td, std::vector<CopyDesc> v) {
+    int64_t t = 0;
+    std::vector<CopyDesc> d;
+    double bytes = 0;
+    for (auto& c : v) {
+      if (c.rows <= 0 || c.cols <= 0) continue;
+      c.tile_begin = t;
+      t += cdiv(int64_t(c.rows) * c.cols, kCopyTile);
+      bytes += double(c.rows) * c.cols * (esize(ts) + esize(td) * (c.beta != 0.0 ? 2 : 1));
+      d.push_back(c);
+    }
+    if (d.empty()) return;
+    Launch L;
+    L.kind = K_COPY; L.cat = cat; L.t[0] = ts; L.t[1] = td;
+    L.off = blob.add(d); L.count = int(d.size()); L.tiles = t; L.bytes = bytes;
+    pr.l.push_back(L);
+  }
+
+  // Blocked LU with partial pivoting of a batch of square blocks.
+  void lu(Proc& pr, Cat panel_cat, Cat row_cat, Cat upd_cat, Cat misc_cat, DT t, const std::vector<LuDesc>& v) {
+    if (v.empty()) return;
+    int mmax = 0;
+    for (auto& x : v) mmax = std::max(mmax, x.m);
+    size_t off = blob.add(v);
+    Launch I;
+    I.kind = K_LU_INIT; I.cat = misc_cat; I.off = off; I.count = int(v.size());
+    pr.l.push_back(I);
+    for (int k0 = 0; k0 < mmax; k0 += kLuNB) {
+      Launch Pn;
+      Pn.kind = K_LU_PANEL; Pn.cat = panel_cat; Pn.t[0] = t; Pn.off = off; Pn.count = int(v.size());
+      Pn.k0 = k0; Pn.rows_max = mmax - k0;
+      double pf = 0;
+      for (auto& x : v)
+        if (x.m > k0) { int nb = std::min(kLuNB, x.m - k0); pf += double(x.m - k0) * nb * nb; }
+      Pn.flops = pf;
+      pr.l.push_back(Pn);
+      if (mmax - k0 - kLuNB > 0) {
+        Launch R;
+        R.kind = K_LU_ROW; R.cat = row_cat; R.t[0] = t; R.off = off; R.count = int(v.size());
+        R.k0 = k0; R.mmax = mmax;
+        pr.l.push_back(R);
+        std::vector<G> gs;
+        for (auto& x : v) {
+          int rest = x.m - k0 - kLuNB;
+          if (rest <= 0) continue;
+          char* A = (char*)x.A;
+          size_t es = esize(t);
+          G g;
+          g.A = A + es * (size_t(k0 + kLuNB) + size_t(k0) * x.lda);
+          g.lda = x.lda;
+          g.B = A + es * (size_t(k0) + size_t(k0 + kLuNB) * x.lda);
+          g.ldb = x.lda;
+          g.Cout = A + es * (size_t(k0 + kLuNB) + size_t(k0 + kLuNB) * x.lda);
+          g.Cin = g.Cout;
+          g.ldcin = g.ldcout = x.lda;
+          g.m = rest; g.n = rest; g.k = kLuNB;
+          g.alpha = -1.0; g.beta = 1.0;
+          gs.push_back(g);
+        }
+        gemm(pr, upd_cat, t, t, t, t, false, 3, gs);
+      }
+    }
+    Launch Pm;
+    Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
+    pr.l.push_back(Pm);
+  }
+
+  void trsm(Proc& pr, Cat cat, DT tlu, DT tin, DT tout, std::vector<TrsmDesc> v) {
+    int64_t t = 0;
+    std::vector<TrsmDesc> d;
+    double flops = 0, bytes = 0;
+    for (auto& x : v) {
+      if (x.m <= 0 || x.ncols <= 0) continue;
+      x.tile_begin = t;
+      t += cdiv(x.ncols, kTrsmCB);
+      flops += 2.0 * double(x.m) * x.m * x.ncols;
+      bytes += double(x.m) * x.m * esize(tlu) + double(x.m) * x.ncols * (esize(tin) + esize(tout));
+      d.push_back(x);
+    }
+    if (d.empty()) return;
+    Launch L;
+    L.kind = K_TRSM; L.cat = cat; L.t[0] = tlu; L.t[1] = tin; L.t[2] = tout;
+    L.off = blob.add(d); L.count = int(d.size()); L.tiles = t; L.flops = flops; L.bytes = bytes;
+    pr.l.push_back(L);
+  }
+
+  void norms(Proc& pr, std::vector<NormDesc> v) {
+    int64_t t = 0;
+    size_t part = 0;
+    for (auto& x : v) {
+      x.nchunks = int(cdiv(x.n, kNormChunk));
+      x.tile_begin = t;
+      x.partial = (double*)(uintptr_t)(part * 8);
+      t += x.nchunks;
+      part += x.nchunks;
+    }
+    scratch_need = std::max(scratch_need, part);
+    for (auto& x : v) x.partial = (double*)(P->scratch.p + (uintptr_t)x.partial);
+    size_t off = blob.add(v);
+    Launch L;
+    L.kind = K_NORM; L.cat = CAT_NORM; L.off = off; L.count = int(v.size()); L.tiles = t;
+    pr.l.push_back(L);
+    Launch F;
+    F.kind = K_NORM_FIN; F.cat = CAT_NORM; F.off = off; F.count = int(v.size());
+    pr.l.push_back(F);
+  }
+};
+
+template <class T> T* at(hodlr_plan_s* P, size_t byte_off) { return (T*)(P->ws.p + byte_off); }
+
+// Workspace layout: one allocation, every region 256-byte aligned.
+struct Layout {
+  size_t top = 0;
+  size_t take(size_t bytes) {
+    size_t o = (top + 255) & ~size_t(255);
+    top = o + bytes;
+    return o;
+  }
+};
+
+int layout_plan(hodlr_plan_s* P, Layout& L) {
+  const size_t we = esize(P->wt);
+  int node_count = 0;
+  for (SysState& S : P->sys) {
+    const Tree& t = S.tree;
+    const int64_t n = t.n, c = t.nrhs;
+    S.ldw = n;
+    S.xws = L.take(size_t(n) * t.xcols() * we);
+    S.yws = L.take(size_t(n) * c * we);
+    S.rbuf = L.take(size_t(n) * c * 8);
+    S.lpiv = L.take(size_t(n) * 4);
+    S.lperm = L.take(size_t(n) * 4);
+    size_t lu_elems = 0, z_elems = 0, k_elems = 0;
+    S.lu_off.resize(t.leaves.size());
+    S.z_off.resize(t.leaves.size());
+    S.kofs.resize(t.leaves.size());
+    for (size_t j = 0; j < t.leaves.size(); ++j) {
+      const TreeNode& x = t.nodes[t.leaves[j]];
+      S.lu_off[j] = lu_elems;
+      lu_elems += size_t(x.nrow) * x.nrow;
+      S.z_off[j] = z_elems;
+      z_elems += size_t(t.rsum[x.depth]) * c;
+      S.kofs[j] = S.s.klayout == HODLR_K_STACKED ? size_t(x.row_beg) : k_elems;
+      k_elems += size_t(x.nrow) * x.nrow;
+    }
+    S.lu = L.take(lu_elems * we);
+    S.zbuf = L.take(std::max<size_t>(z_elems, 1) * 8);
+    const size_t nn = t.nodes.size();
+    S.vtd.assign(nn, 0); S.vtu.assign(nn, 0); S.wmv.assign(nn, 0);
+    S.sbuf.assign(nn, 0); S.eta.assign(nn, 0); S.spiv.assign(nn, 0); S.sperm.assign(nn, 0);
+    S.sinfo.assign(nn, -1);
+    for (size_t i = 1; i < nn; ++i) {
+      const TreeNode& x = t.nodes[i];
+      const int r = t.rank(x.depth);
+      const int64_t nd = t.col_beg(x.depth);
+      S.vtd[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
+      S.vtu[i] = L.take(size_t(r) * r * 8);
+      S.wmv[i] = L.take(size_t(r) * c * 8);
+    }
+    for (size_t i = 0; i < nn; ++i) {
+      const TreeNode& x = t.nodes[i];
+      if (x.is_leaf()) continue;
+      const int r = t.rank(x.depth + 1);
+      const int64_t nd = t.col_beg(x.depth + 1);
+      S.sbuf[i] = L.take(size_t(r) * r * 8);
+      S.eta[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
+      S.spiv[i] = L.take(size_t(r) * 4);
+      S.sperm[i] = L.take(size_t(r) * 4);
+      S.sinfo[i] = node_count++;
+    }
+    S.bnorm = L.take(size_t(c) * 8);
+    S.rnorm = L.take(size_t(c) * 8);
+  }
+  P->nnodes = node_count;
+  return 0;
+}
+
+void build_all(hodlr_plan_s* P, Builder& B) {
+  const DT wt = P->wt;
+  const size_t we = esize(wt);
+  const int H = P->height;
+  P->reduce.assign(H + 1, Proc());
+  P->node.assign(H + 1, Proc());
+  P->bcast.assign(H + 1, Proc());
+  P->reduce_s.assign(H + 1, Proc());
+  P->node_s.assign(H + 1, Proc());
+  P->bcast_s.assign(H + 1, Proc());
+  int leaf_base = 0;
+
+  // ---------------------------------------------------------------- leaves
+  {
+    std::vector<CopyDesc> kc;
+    std::vector<LuDesc> lus;
+    std::vector<TrsmDesc> tr, trs;
+    for (SysState& S : P->sys) {
+      const Tree& t = S.tree;
+      for (size_t j = 0; j < t.leaves.size(); ++j) {
+        const TreeNode& x = t.nodes[t.leaves[j]];
+        const int m = int(x.nrow);
+        CopyDesc c{};
+        c.src = S.s.K + S.kofs[j];
+        c.lds = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : x.nrow;
+        c.dst = P->ws.p + S.lu + S.lu_off[j] * we;
+        c.ldd = m;
+        c.rows = c.cols = m;
+        c.alpha = 1.0;
+        kc.push_back(c);
+        LuDesc l{};
+        l.A = P->ws.p + S.lu + S.lu_off[j] * we;
+        l.lda = m;
+        l.m = m;
+        l.ipiv = at<int32_t>(P, S.lpiv) + x.row_beg;
+        l.perm = at<int32_t>(P, S.lperm) + x.row_beg;
+        l.info = P->leaf_info + leaf_base + j;
+        lus.push_back(l);
+        TrsmDesc d{};
+        d.LU = l.A; d.ldlu = m; d.perm = l.perm;
+        d.Xin = S.s.X + x.row_beg; d.ldin = S.s.ldx;
+        d.Xout = P->ws.p + S.xws + size_t(x.row_beg) * we; d.ldout = S.ldw;
+        d.m = m; d.ncols = int(t.leaf_nrhs(t.leaves[j]));
+        tr.push_back(d);
+        TrsmDesc e = d;
+        e.Xin = at<double>(P, S.rbuf) + x.row_beg; e.ldin = S.ldw;
+        e.Xout = P->ws.p + S.yws + size_t(x.row_beg) * we; e.ldout = S.ldw;
+        e.ncols = t.nrhs;
+        trs.push_back(e);
+      }
+      leaf_base += int(t.leaves.size());
+    }
+    B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, kc);
+    B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus);
+    B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr);
+    B.trsm(P->solve_only_leaf, CAT_LEAF_TRSM, wt, F64, wt, trs);
+  }
+
+  // ---------------------------------------------------------------- heights
+  for (int h = 1; h <= H; ++h) {
+    for (int mode = 0; mode < 2; ++mode) {   // 0: fused factor+solve, 1: solve-only
+      std::vector<Builder::G> red, ng1, ng2, bc;
+      std::vector<CopyDesc> sform;
+      std::vector<LuDesc> slu;
+      std::vector<TrsmDesc> str;
+      for (SysState& S : P->sys) {
+        const Tree& t = S.tree;
+        const int64_t c = t.nrhs;
+        char* X = P->ws.p + (mode == 0 ? S.xws : S.yws);
+        char* F = P->ws.p + S.xws;   // factor columns
+        for (size_t pi = 0; pi < t.nodes.size(); ++pi) {
+          const TreeNode& Pn = t.nodes[pi];
+          if (Pn.is_leaf() || Pn.height != h) continue;
+          const int d = Pn.depth + 1;
+          const int r = t.rank(d);
+          const int64_t ucol = t.col_beg(d), vcol = t.rsum[d - 1];
+          const int nd = int(mode == 0 ? ucol : c);
+          const int ch[2] = {Pn.child[0], Pn.child[1]};
+          double* vtd[2] = {at<double>(P, S.vtd[ch[0]]), at<double>(P, S.vtd[ch[1]])};
+          double* vtu[2] = {at<double>(P, S.vtu[ch[0]]), at<double>(P, S.vtu[ch[1]])};
+          double* eta0 = at<double>(P, S.eta[pi]);
+          double* Sm = at<double>(P, S.sbuf[pi]);
+          for (int k = 0; k < 2; ++k) {
+            const TreeNode& C = t.nodes[ch[k]];
+            Builder::G g{};
+            g.A = S.s.V + C.row_beg + vcol * S.s.ldv; g.lda = S.s.ldv;
+            g.B = X + size_t(C.row_beg) * we; g.ldb = S.ldw;
+            g.Cout = vtd[k]; g.ldcout = r; g.Cin = nullptr; g.ldcin = r;
+            g.m = r; g.n = nd; g.k = int(C.nrow); g.alpha = 1.0; g.beta = 0.0;
+            red.push_back(g);
+            if (mode == 0) {
+              Builder::G u = g;
+              u.B = X + (size_t(C.row_beg) + size_t(ucol) * S.ldw) * we;
+              u.Cout = vtu[k]; u.n = r;
+              red.push_back(u);
+            }
+          }
+          // node: VTd1 -= VTu1 VTd0
+          Builder::G a{};
+          a.A = vtu[1]; a.lda = r; a.B = vtd[0]; a.ldb = r; a.Cin = vtd[1]; a.ldcin = r; a.Cout = vtd[1]; a.ldcout = r;
+          a.m = r; a.n = nd; a.k = r; a.alpha = -1.0; a.beta = 1.0;
+          ng1.push_back(a);
+          if (mode == 0) {
+            Builder::G s{};
+            s.A = vtu[1]; s.lda = r; s.B = vtu[0]; s.ldb = r; s.Cout = Sm; s.ldcout = r; s.Cin = nullptr; s.ldcin = r;
+            s.m = r; s.n = r; s.k = r; s.alpha = 1.0; s.beta = 0.0;
+            ng1.push_back(s);
+            CopyDesc cd{};
+            cd.src = Sm; cd.lds = r; cd.dst = Sm; cd.ldd = r; cd.rows = cd.cols = r;
+            cd.alpha = -1.0; cd.beta = 0.0; cd.diag = 1.0;
+            sform.push_back(cd);
+            LuDesc l{};
+            l.A = Sm; l.lda = r; l.m = r;
+            l.ipiv = at<int32_t>(P, S.spiv[pi]);
+            l.perm = at<int32_t>(P, S.sperm[pi]);
+            l.info = P->node_info + S.sinfo[pi];
+            slu.push_back(l);
+          }
+          TrsmDesc td{};
+          td.LU = Sm; td.ldlu = r; td.perm = at<int32_t>(P, S.sperm[pi]);
+          td.Xin = vtd[1]; td.ldin = r; td.Xout = eta0; td.ldout = r;
+          td.m = r; td.ncols = nd;
+          str.push_back(td);
+          Builder::G b{};
+          b.A = vtu[0]; b.lda = r; b.B = eta0; b.ldb = r; b.Cin = vtd[0]; b.ldcin = r; b.Cout = vtd[0]; b.ldcout = r;
+          b.m = r; b.n = nd; b.k = r; b.alpha = -1.0; b.beta = 1.0;
+          ng2.push_back(b);
+          // broadcast: d0 -= u0 eta0, d1 -= u1 eta1 (eta1 in the VTd0 slot)
+          const double* etas[2] = {eta0, vtd[0]};
+          for (int k = 0; k < 2; ++k) {
+            const TreeNode& C = t.nodes[ch[k]];
+            Builder::G g{};
+            g.A = F + (size_t(C.row_beg) + size_t(ucol) * S.ldw) * we; g.lda = S.ldw;
+            g.B = etas[k]; g.ldb = r;
+            g.Cout = X + size_t(C.row_beg) * we; g.Cin = X + size_t(C.row_beg) * we; g.ldcin = g.ldcout = S.ldw;
+            g.m = int(C.nrow); g.n = nd; g.k = r; g.alpha = -1.0; g.beta = 1.0;
+            bc.push_back(g);
+          }
+        }
+      }
+      Proc& pr = mode == 0 ? P->reduce[h] : P->reduce_s[h];
+      Proc& pn = mode == 0 ? P->node[h] : P->node_s[h];
+      Proc& pb = mode == 0 ? P->bcast[h] : P->bcast_s[h];
+      B.gemm(pr, CAT_REDUCE, F64, wt, F64, wt, true, 1, red, 2048);
+      B.gemm(pn, CAT_NODE_GEMM, F64, F64, F64, F64, false, 0, ng1);
+      if (mode == 0) {
+        B.copy(pn, CAT_NODE_LU, F64, F64, sform);
+        B.lu(pn, CAT_NODE_LU, CAT_NODE_LU, CAT_NODE_LU, CAT_NODE_LU, F64, slu);
+      }
+      B.trsm(pn, CAT_NODE_TRSM, F64, F64, F64, str);
+      B.gemm(pn, CAT_NODE_GEMM, F64, F64, F64, F64, false, 0, ng2);
+      B.gemm(pb, CAT_BCAST, wt, F64, wt, wt, false, 2, bc);
+    }
+  }
+
+  // ---------------------------------------------------------------- extract / update
+  {
+    std::vector<CopyDesc> ex, up, in;
+    for (SysState& S : P->sys) {
+      CopyDesc c{};
+      c.src = P->ws.p + S.xws; c.lds = S.ldw; c.dst = S.s.x; c.ldd = S.s.ldxs;
+      c.rows = int(S.tree.n); c.cols = S.tree.nrhs; c.alpha = 1.0;
+      ex.push_back(c);
+      CopyDesc u = c;
+      u.src = P->ws.p + S.yws; u.beta = 1.0;
+      up.push_back(u);
+    }
+    B.copy(P->extract, CAT_COPY, wt, F64, ex);
+    B.copy(P->update, CAT_COPY, wt, F64, up);
+  }
+
+  // ---------------------------------------------------------------- residual r = b - A x
+  {
+    std::vector<Builder::G> red, lk, lu;
+    std::vector<CopyDesc> gz;
+    std::vector<NormDesc> nr;
+    for (SysState& S : P->sys) {
+      const Tree& t = S.tree;
+      const int64_t c = t.nrhs;
+      double* R = at<double>(P, S.rbuf);
+      double* Z = at<double>(P, S.zbuf);
+      for (size_t i = 1; i < t.nodes.size(); ++i) {
+        const TreeNode& x = t.nodes[i];
+        const int r = t.rank(x.depth);
+        Builder::G g{};
+        g.A = S.s.V + x.row_beg + t.rsum[x.depth - 1] * S.s.ldv; g.lda = S.s.ldv;
+        g.B = S.s.x + x.row_beg; g.ldb = S.s.ldxs;
+        g.Cout = at<double>(P, S.wmv[i]); g.ldcout = r;
+        g.m = r; g.n = int(c); g.k = int(x.nrow); g.alpha = 1.0; g.beta = 0.0;
+        red.push_back(g);
+      }
+      for (size_t j = 0; j < t.leaves.size(); ++j) {
+        const int id = t.leaves[j];
+        const TreeNode& x = t.nodes[id];
+        Builder::G g{};
+        g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : x.nrow;
+        g.B = S.s.x + x.row_beg; g.ldb = S.s.ldxs;
+        g.Cin = S.s.X + x.row_beg; g.ldcin = S.s.ldx;
+        g.Cout = R + x.row_beg; g.ldcout = S.ldw;
+        g.m = int(x.nrow); g.n = int(c); g.k = int(x.nrow); g.alpha = -1.0; g.beta = 1.0;
+        lk.push_back(g);
+        const int64_t Rd = t.rsum[x.depth];
+        if (Rd == 0) continue;
+        // Z_j rows of depth l: w of the sibling of the depth-l ancestor
+        int a = id;
+        while (a != 0) {
+          const TreeNode& an = t.nodes[a];
+          const TreeNode& par = t.nodes[an.parent];
+          const int sib = par.child[0] == a ? par.child[1] : par.child[0];
+          const int r = t.rank(an.depth);
+          CopyDesc cd{};
+          cd.src = at<double>(P, S.wmv[sib]); cd.lds = r;
+          cd.dst = Z + S.z_off[j] + t.rsum[an.depth - 1]; cd.ldd = Rd;
+          cd.rows = r; cd.cols = int(c); cd.alpha = 1.0;
+          gz.push_back(cd);
+          a = an.parent;
+        }
+        Builder::G u{};
+        u.A = S.s.X + x.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
+        u.B = Z + S.z_off[j]; u.ldb = Rd;
+        u.Cin = R + x.row_beg; u.ldcin = S.ldw; u.Cout = R + x.row_beg; u.ldcout = S.ldw;
+        u.m = int(x.nrow); u.n = int(c); u.k = int(Rd); u.alpha = -1.0; u.beta = 1.0;
+        lu.push_back(u);
+      }
+      for (int64_t j = 0; j < c; ++j) {
+        NormDesc d{};
+        d.x = R + j * S.ldw; d.n = t.n; d.out = at<double>(P, S.rnorm) + j;
+        nr.push_back(d);
+      }
+    }
+    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, true, 1, red, 4096);
+    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
+    B.copy(P->residual, CAT_MATVEC, F64, F64, gz);
+    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
+    B.norms(P->residual, nr);
+  }
+}
+
+// ------------------------------------------------------------------ execution
+hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
+  const void* d = P->descs.p + L.off;
+  hipStream_t s = P->stream;
+  switch (L.kind) {
+    case K_GEMM: return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, (const GemmDesc*)d, L.count, L.tiles, s);
+    case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
+    case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s);
+    case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
+    case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
+    case K_LU_ROW: return launch_lu_rowblock(L.t[0], (const LuDesc*)d, L.count, L.k0, L.mmax, s);
+    case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
+    case K_TRSM: return launch_trsm(L.t[0], L.t[1], L.t[2], (const TrsmDesc*)d, L.count, L.tiles, s);
+    case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s);
+    case K_NORM_FIN: return launch_norm_finish((const NormDesc*)d, L.count, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+int run(hodlr_plan_s* P, const Proc& pr) {
+  for (const Launch& L : pr.l) {
+    if (P->profile) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return HODLR_ERR_HIP;
+      (void)hipEventRecord(a, P->stream);
+      if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
+      (void)hipEventRecord(b, P->stream);
+      if (hipEventSynchronize(b) != hipSuccess) return HODLR_ERR_HIP;
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a, b);
+      P->prof_ms[L.cat] += ms;
+      P->prof_n[L.cat] += 1;
+      P->prof_flops[L.cat] += L.flops;
+      P->prof_bytes[L.cat] += L.bytes;
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    } else if (dispatch(P, L) != hipSuccess) {
+      return HODLR_ERR_HIP;
+    }
+  }
+  return HODLR_OK;
+}
+
+double proc_flops(const Proc& p) {
+  double f = 0;
+  for (auto& l : p.l) f += l.flops;
+  return f;
+}
+
+void fill_info(hodlr_plan_s* P) {
+  hodlr_plan_info_t& I = P->info;
+  I.nsys = int(P->sys.size());
+  I.precision = P->precision;
+  I.height = P->height;
+  I.nleaves = P->nleaves;
+  I.workspace_bytes = int64_t(P->ws.bytes + P->descs.bytes + P->scratch.bytes);
+  // algorithmic work (flop and byte model of one fused pass)
+  const double we = double(esize(P->wt));
+  double fl = 0, fr = 0, fn = 0, fb = 0, bl = 0, br = 0, bb = 0, fm = 0, bm = 0, fs = 0, bs = 0;
+  for (SysState& S : P->sys) {
+    const Tree& t = S.tree;
+    const double c = t.nrhs;
+    for (int id : t.leaves) {
+      const TreeNode& x = t.nodes[id];
+      const double m = double(x.nrow), nr = double(t.leaf_nrhs(id)), Rd = double(t.rsum[x.depth]);
+      fl += 2.0 / 3.0 * m * m * m + 2.0 * m * m * nr;
+      bl += we * (m * m + 2.0 * m * nr);
+      fm += 2.0 * m * m * c + 2.0 * m * Rd * c;
+      bm += 8.0 * (m * m + m * Rd);
+      fs += 2.0 * m * m * c;
+      bs += we * m * m;
+    }
+    for (const TreeNode& Pn : t.nodes) {
+      if (Pn.is_leaf()) continue;
+      const int d = Pn.depth + 1;
+      const double r = t.rank(d), nd = double(t.col_beg(d));
+      for (int k = 0; k < 2; ++k) {
+        const double n = double(t.nodes[Pn.child[k]].nrow);
+        fr += 2.0 * n * r * (nd + r);
+        br += we * n * (nd + r) + 8.0 * n * r;
+        fb += 2.0 * n * r * nd;
+        bb += we * n * (r + 2.0 * nd);
+        fm += 2.0 * 2.0 * n * r * c;
+        bm += 8.0 * n * r * 2.0;
+        fs += 2.0 * 2.0 * n * r * c;
+        bs += 8.0 * n * r + we * n * r;
+      }
+      fn += 8.0 / 3.0 * r * r * r + 6.0 * r * r * nd;
+    }
+    bm += 8.0 * 3.0 * t.n * c;
+    bs += we * 3.0 * t.n * c;
+  }
+  I.flops_leaf = fl; I.flops_reduce = fr; I.flops_node = fn; I.flops_bcast = fb;
+  I.flops_factor = fl + fr + fn + fb;
+  I.bytes_leaf = bl; I.bytes_reduce = br; I.bytes_bcast = bb;
+  I.flops_matvec = fm; I.bytes_matvec = bm;
+  I.flops_solve_only = fs; I.bytes_solve_only = bs;
+  I.launches_factor = int(P->factor.l.size());
+}
+
+int check_system(const hodlr_system_t& s) {
+  if (s.n <= 0 || s.threshold <= 0 || s.nrhs <= 0 || s.nranks <= 0 || !s.ranks) return HODLR_ERR_ARG;
+  if (!s.X || !s.V || !s.K || !s.x) return HODLR_ERR_ARG;
+  if (s.ldx < s.n || s.ldv < s.n || s.ldxs < s.n) return HODLR_ERR_ARG;
+  if (s.klayout != HODLR_K_PACKED && s.klayout != HODLR_K_STACKED) return HODLR_ERR_ARG;
+  if (s.klayout == HODLR_K_STACKED && s.ldk < s.n) return HODLR_ERR_ARG;
+  if (s.n > (int64_t(1) << 31) - 1) return HODLR_ERR_UNSUPPORTED;
+  return HODLR_OK;
+}
+
+void free_plan(hodlr_plan_s* P) {
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  if (P->ws.p) (void)hipFree(P->ws.p);
+  if (P->descs.p) (void)hipFree(P->descs.p);
+  if (P->scratch.p) (void)hipFree(P->scratch.p);
+}
+
+int sync_stream(hodlr_plan_s* P) {
+  return hipStreamSynchronize(P->stream) == hipSuccess ? HODLR_OK : HODLR_ERR_HIP;
+}
+
+int run_factor(hodlr_plan_s* P) {
+  if (P->use_graph && !P->profile) {
+    if (!P->graph_exec) {
+      hipGraph_t g;
+      if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
+      int rc = run(P, P->factor);
+      hipError_t e = hipStreamEndCapture(P->stream, &g);
+      if (rc || e != hipSuccess) return HODLR_ERR_HIP;
+      if (hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        return HODLR_ERR_HIP;
+      }
+      hipGraphDestroy(g);
+    }
+    if (hipGraphLaunch(P->graph_exec, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+  } else {
+    int rc = run(P, P->factor);
+    if (rc) return rc;
+  }
+  P->factored = true;
+  return HODLR_OK;
+}
+
+int read_relres(hodlr_plan_s* P, double* relres, double* mx) {
+  double worst = 0;
+  int cmax = 0;
+  for (auto& S : P->sys) cmax = std::max(cmax, S.tree.nrhs);
+  for (size_t s = 0; s < P->sys.size(); ++s) {
+    SysState& S = P->sys[s];
+    const int c = S.tree.nrhs;
+    std::vector<double> rn(c), bn(c);
+    if (hipMemcpyAsync(rn.data(), P->ws.p + S.rnorm, c * 8, hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
+        hipMemcpyAsync(bn.data(), P->ws.p + S.bnorm, c * 8, hipMemcpyDeviceToHost, P->stream) != hipSuccess)
+      return HODLR_ERR_HIP;
+    if (sync_stream(P)) return HODLR_ERR_HIP;
+    for (int j = 0; j < c; ++j) {
+      double v = bn[j] > 0 ? rn[j] / bn[j] : rn[j];
+      if (!(v == v)) v = INFINITY;
+      if (relres) relres[s * cmax + j] = v;
+      worst = std::max(worst, v);
+    }
+  }
+  if (mx) *mx = worst;
+  return HODLR_OK;
+}
+
+}  // namespace
+
+// ======================================================================= C-ABI
+extern "C" {
+
+const char* hodlr_version(void) { return "fastsolver_amd 0.1 (gfx950)"; }
+
+const char* hodlr_error_string(int32_t code) {
+  switch (code) {
+    case HODLR_OK: return "ok";
+    case HODLR_ERR_ARG: return "invalid argument";
+    case HODLR_ERR_TREE: return "invalid HODLR tree (sizes, ranks, or leaf rows <= rank)";
+    case HODLR_ERR_UNSUPPORTED: return "leaf size or rank above the supported 1024";
+    case HODLR_ERR_HIP: return "HIP runtime error";
+    case HODLR_ERR_NOMEM: return "device allocation failed";
+    case HODLR_ERR_SINGULAR: return "zero pivot in a leaf block or a Schur complement";
+    case HODLR_ERR_NOTCONVERGED: return "refinement did not reach the tolerance";
+  }
+  return "unknown error";
+}
+
+const char* hodlr_profile_category(int32_t i) { return (i >= 0 && i < CAT_N) ? kCatNames[i] : ""; }
+
+int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t nsys, int32_t precision,
+                      void* stream) {
+  if (!plan || !systems || nsys <= 0) return HODLR_ERR_ARG;
+  if (precision != HODLR_PREC_FP64 && precision != HODLR_PREC_FP32) return HODLR_ERR_ARG;
+  *plan = nullptr;
+  std::unique_ptr<hodlr_plan_s> P(new hodlr_plan_s());
+  P->precision = precision;
+  P->wt = precision == HODLR_PREC_FP64 ? F64 : F32;
+  P->stream = (hipStream_t)stream;
+  P->sys.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    int rc = check_system(systems[s]);
+    if (rc) return rc;
+    SysState& S = P->sys[s];
+    S.s = systems[s];
+    S.ranks.assign(systems[s].ranks, systems[s].ranks + systems[s].nranks);
+    S.s.ranks = S.ranks.data();
+    if (build_tree(S.s.n, S.s.threshold, S.s.nrhs, S.s.nranks, S.ranks.data(), S.tree) != TREE_OK)
+      return HODLR_ERR_TREE;
+    for (const TreeNode& x : S.tree.nodes) {
+      if (x.is_leaf() && x.nrow > kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+      if (!x.is_leaf() && S.tree.rank(x.depth + 1) > kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+    }
+    if (int64_t(S.tree.xcols()) * S.tree.n > (int64_t(1) << 40)) return HODLR_ERR_UNSUPPORTED;
+    P->height = std::max(P->height, S.tree.height);
+    P->nleaves += int(S.tree.leaves.size());
+  }
+  // workspace
+  Layout L;
+  if (layout_plan(P.get(), L)) return HODLR_ERR_ARG;
+  size_t linfo = L.take(size_t(P->nleaves) * 4);
+  size_t ninfo = L.take(size_t(std::max(P->nnodes, 1)) * 4);
+  P->ws.bytes = L.top;
+  if (hipMalloc((void**)&P->ws.p, P->ws.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
+  P->leaf_info = (int32_t*)(P->ws.p + linfo);
+  P->node_info = (int32_t*)(P->ws.p + ninfo);
+  // scratch must exist before descriptors reference it: size it with a dry run
+  {
+    Builder dry{P.get()};
+    build_all(P.get(), dry);
+    P->scratch_doubles = std::max<size_t>(dry.scratch_need, 1);
+    if (hipMalloc((void**)&P->scratch.p, P->scratch_doubles * 8) != hipSuccess) {
+      free_plan(P.get());
+      return HODLR_ERR_NOMEM;
+    }
+    P->scratch.bytes = P->scratch_doubles * 8;
+  }
+  Builder B{P.get()};
+  P->leaf = Proc(); P->extract = Proc(); P->residual = Proc(); P->update = Proc(); P->solve_only_leaf = Proc();
+  build_all(P.get(), B);
+  // the fused pass
+  P->factor = Proc();
+  P->factor.append(P->leaf);
+  for (int h = 1; h <= P->height; ++h) {
+    P->factor.append(P->reduce[h]);
+    P->factor.append(P->node[h]);
+    P->factor.append(P->bcast[h]);
+  }
+  P->factor.append(P->extract);
+  P->descs.bytes = std::max<size_t>(B.blob.data.size(), 256);
+  if (hipMalloc((void**)&P->descs.p, P->descs.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
+  // descriptors embed the final device addresses: upload once
+  if (hipMemcpy(P->descs.p, B.blob.data.data(), B.blob.data.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    free_plan(P.get());
+    return HODLR_ERR_HIP;
+  }
+  fill_info(P.get());
+  // norms of b (used by every residual)
+  {
+    Builder nb{P.get()};
+    Proc pr;
+    std::vector<NormDesc> v;
+    for (SysState& S : P->sys)
+      for (int j = 0; j < S.tree.nrhs; ++j) {
+        NormDesc d{};
+        d.x = S.s.X + int64_t(j) * S.s.ldx;
+        d.n = S.tree.n;
+        d.out = at<double>(P.get(), S.bnorm) + j;
+        v.push_back(d);
+      }
+    nb.norms(pr, v);
+    Buf tmp;
+    tmp.bytes = nb.blob.data.size();
+    if (hipMalloc((void**)&tmp.p, tmp.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
+    hipMemcpy(tmp.p, nb.blob.data.data(), tmp.bytes, hipMemcpyHostToDevice);
+    char* save = P->descs.p;
+    P->descs.p = tmp.p;
+    int rc = run(P.get(), pr);
+    P->descs.p = save;
+    hipError_t e = hipStreamSynchronize(P->stream);
+    hipFree(tmp.p);
+    if (rc || e != hipSuccess) { free_plan(P.get()); return HODLR_ERR_HIP; }
+  }
+  *plan = P.release();
+  return HODLR_OK;
+}
+
+int hodlr_plan_destroy(hodlr_plan_t P) {
+  if (!P) return HODLR_ERR_ARG;
+  hipStreamSynchronize(P->stream);
+  free_plan(P);
+  delete P;
+  return HODLR_OK;
+}
+
+int hodlr_plan_query(hodlr_plan_t P, hodlr_plan_info_t* info) {
+  if (!P || !info) return HODLR_ERR_ARG;
+  *info = P->info;
+  return HODLR_OK;
+}
+
+int hodlr_factor_solve(hodlr_plan_t P) {
+  if (!P) return HODLR_ERR_ARG;
+  return run_factor(P);
+}
+
+int hodlr_leaf_solve(hodlr_plan_t P) { return P ? run(P, P->leaf) : HODLR_ERR_ARG; }
+int hodlr_level_reduce(hodlr_plan_t P, int32_t h) {
+  if (!P || h < 1 || h > P->height) return HODLR_ERR_ARG;
+  return run(P, P->reduce[h]);
+}
+int hodlr_level_node_solve(hodlr_plan_t P, int32_t h) {
+  if (!P || h < 1 || h > P->height) return HODLR_ERR_ARG;
+  return run(P, P->node[h]);
+}
+int hodlr_level_broadcast(hodlr_plan_t P, int32_t h) {
+  if (!P || h < 1 || h > P->height) return HODLR_ERR_ARG;
+  int rc = run(P, P->bcast[h]);
+  if (!rc && h == P->height) P->factored = true;
+  return rc;
+}
+int hodlr_extract_solution(hodlr_plan_t P) { return P ? run(P, P->extract) : HODLR_ERR_ARG; }
+
+static int solve_only_internal(hodlr_plan_t P) {
+  int rc = run(P, P->solve_only_leaf);
+  for (int h = 1; h <= P->height && !rc; ++h) {
+    rc = run(P, P->reduce_s[h]);
+    if (!rc) rc = run(P, P->node_s[h]);
+    if (!rc) rc = run(P, P->bcast_s[h]);
+  }
+  return rc;
+}
+
+static int stage_copy(hodlr_plan_t P, const double* const* src, const int64_t* lds, bool to_rbuf,
+                      double* const* dst, const int64_t* ldd) {
+  // host-driven 2D copies between user arrays and the plan's residual buffer
+  for (size_t s = 0; s < P->sys.size(); ++s) {
+    SysState& S = P->sys[s];
+    const size_t w = size_t(S.tree.n) * 8;
+    hipError_t e;
+    if (to_rbuf)
+      e = hipMemcpy2DAsync(P->ws.p + S.rbuf, S.ldw * 8, src[s], lds[s] * 8, w, S.tree.nrhs, hipMemcpyDeviceToDevice,
+                           P->stream);
+    else
+      e = hipMemcpy2DAsync(dst[s], ldd[s] * 8, P->ws.p + S.rbuf, S.ldw * 8, w, S.tree.nrhs, hipMemcpyDeviceToDevice,
+                           P->stream);
+    if (e != hipSuccess) return HODLR_ERR_HIP;
+  }
+  return HODLR_OK;
+}
+
+int hodlr_solve_only(hodlr_plan_t P, const double* const* rhs, const int64_t* ldr, double* const* out,
+                     const int64_t* ldo) {
+  if (!P || !rhs || !ldr || !out || !ldo) return HODLR_ERR_ARG;
+  if (!P->factored) return HODLR_ERR_ARG;
+  int rc = stage_copy(P, rhs, ldr, true, nullptr, nullptr);
+  if (!rc) rc = solve_only_internal(P);
+  if (rc) return rc;
+  // out = Y[:, 0:c] (converted to double)
+  for (size_t s = 0; s < P->sys.size(); ++s) {
+    SysState& S = P->sys[s];
+    Builder b{P};
+    Proc pr;
+    CopyDesc c{};
+    c.src = P->ws.p + S.yws; c.lds = S.ldw; c.dst = out[s]; c.ldd = ldo[s];
+    c.rows = int(S.tree.n); c.cols = S.tree.nrhs; c.alpha = 1.0;
+    b.copy(pr, CAT_COPY, P->wt, F64, {c});
+    void* tmp = nullptr;
+    if (hipMalloc(&tmp, b.blob.data.size()) != hipSuccess) return HODLR_ERR_NOMEM;
+    hipMemcpyAsync(tmp, b.blob.data.data(), b.blob.data.size(), hipMemcpyHostToDevice, P->stream);
+    char* save = P->descs.p;
+    P->descs.p = (char*)tmp;
+    rc = run(P, pr);
+    P->descs.p = save;
+    hipStreamSynchronize(P->stream);
+    hipFree(tmp);
+    if (rc) return rc;
+  }
+  return HODLR_OK;
+}
+
+int hodlr_matvec(hodlr_plan_t P, const double* const* x, const int64_t* ldx, double* const* y,
+                 const int64_t* ldy) {
+  if (!P || !x || !ldx || !y || !ldy) return HODLR_ERR_ARG;
+  // y = A x via the residual pipeline with b = 0: build a one-off batch
+  Builder B{P};
+  Proc pr;
+  std::vector<Builder::G> red, lk, lu;
+  std::vector<CopyDesc> gz;
+  for (size_t s = 0; s < P->sys.size(); ++s) {
+    SysState& S = P->sys[s];
+    const Tree& t = S.tree;
+    const int64_t c = t.nrhs;
+    double* Z = at<double>(P, S.zbuf);
+    for (size_t i = 1; i < t.nodes.size(); ++i) {
+      const TreeNode& nd = t.nodes[i];
+      const int r = t.rank(nd.depth);
+      Builder::G g{};
+      g.A = S.s.V + nd.row_beg + t.rsum[nd.depth - 1] * S.s.ldv; g.lda = S.s.ldv;
+      g.B = x[s] + nd.row_beg; g.ldb = ldx[s];
+      g.Cout = at<double>(P, S.wmv[i]); g.ldcout = r;
+      g.m = r; g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
+      red.push_back(g);
+    }
+    for (size_t j = 0; j < t.leaves.size(); ++j) {
+      const int id = t.leaves[j];
+      const TreeNode& nd = t.nodes[id];
+      Builder::G g{};
+      g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : nd.nrow;
+      g.B = x[s] + nd.row_beg; g.ldb = ldx[s];
+      g.Cout = y[s] + nd.row_beg; g.ldcout = ldy[s];
+      g.m = int(nd.nrow); g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
+      lk.push_back(g);
+      const int64_t Rd = t.rsum[nd.depth];
+      if (Rd == 0) continue;
+      int a = id;
+      while (a != 0) {
+        const TreeNode& an = t.nodes[a];
+        const TreeNode& par = t.nodes[an.parent];
+        const int sib = par.child[0] == a ? par.child[1] : par.child[0];
+        const int r = t.rank(an.depth);
+        CopyDesc cd{};
+        cd.src = at<double>(P, S.wmv[sib]); cd.lds = r;
+        cd.dst = Z + S.z_off[j] + t.rsum[an.depth - 1]; cd.ldd = Rd;
+        cd.rows = r; cd.cols = int(c); cd.alpha = 1.0;
+        gz.push_back(cd);
+        a = an.parent;
+      }
+      Builder::G u{};
+      u.A = S.s.X + nd.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
+      u.B = Z + S.z_off[j]; u.ldb = Rd;
+      u.Cin = y[s] + nd.row_beg; u.ldcin = ldy[s]; u.Cout = y[s] + nd.row_beg; u.ldcout = ldy[s];
+      u.m = int(nd.nrow); u.n = int(c); u.k = int(Rd); u.alpha = 1.0; u.beta = 1.0;
+      lu.push_back(u);
+    }
+  }
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red, 4096);
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
+  B.copy(pr, CAT_MATVEC, F64, F64, gz);
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
+  if (B.scratch_need > P->scratch_doubles) return HODLR_ERR_UNSUPPORTED;
+  void* tmp = nullptr;
+  if (hipMalloc(&tmp, B.blob.data.size()) != hipSuccess) return HODLR_ERR_NOMEM;
+  hipMemcpyAsync(tmp, B.blob.data.data(), B.blob.data.size(), hipMemcpyHostToDevice, P->stream);
+  char* save = P->descs.p;
+  P->descs.p = (char*)tmp;
+  int rc = run(P, pr);
+  P->descs.p = save;
+  hipError_t e = hipStreamSynchronize(P->stream);
+  hipFree(tmp);
+  if (rc) return rc;
+  return e == hipSuccess ? HODLR_OK : HODLR_ERR_HIP;
+}
+
+int hodlr_residual(hodlr_plan_t P, double* relres, double* relres_max) {
+  if (!P) return HODLR_ERR_ARG;
+  int rc = run(P, P->residual);
+  if (rc) return rc;
+  return read_relres(P, relres, relres_max);
+}
+
+int hodlr_solve(hodlr_plan_t P, double tol, int32_t maxiter, hodlr_solve_report_t* rep) {
+  if (!P) return HODLR_ERR_ARG;
+  hodlr_solve_report_t R{};
+  int rc = run_factor(P);
+  if (rc) return rc;
+  double rr = 0;
+  rc = run(P, P->residual);
+  if (!rc) rc = read_relres(P, nullptr, &rr);
+  if (rc) return rc;
+  R.relres_first = rr;
+  int it = 0;
+  while (!(rr <= tol) && it < maxiter) {
+    // the residual already sits in the plan's right-hand-side buffer
+    rc = solve_only_internal(P);
+    if (!rc) rc = run(P, P->update);
+    if (!rc) rc = run(P, P->residual);
+    if (!rc) rc = read_relres(P, nullptr, &rr);
+    if (rc) return rc;
+    ++it;
+  }
+  R.iterations = it;
+  R.relres = rr;
+  R.converged = rr <= tol;
+  if (rep) *rep = R;
+  return R.converged ? HODLR_OK : HODLR_ERR_NOTCONVERGED;
+}
+
+int hodlr_status(hodlr_plan_t P, int32_t* leaf_info, int32_t* node_info, int32_t* n_nodes) {
+  if (!P) return HODLR_ERR_ARG;
+  if (sync_stream(P)) return HODLR_ERR_HIP;
+  std::vector<int32_t> li(P->nleaves), ni(std::max(P->nnodes, 1));
+  if (hipMemcpy(li.data(), P->leaf_info, li.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return HODLR_ERR_HIP;
+  if (P->nnodes && hipMemcpy(ni.data(), P->node_info, size_t(P->nnodes) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return HODLR_ERR_HIP;
+  if (leaf_info) memcpy(leaf_info, li.data(), li.size() * 4);
+  if (node_info && P->nnodes) memcpy(node_info, ni.data(), size_t(P->nnodes) * 4);
+  if (n_nodes) *n_nodes = P->nnodes;
+  for (int v : li) if (v) return HODLR_ERR_SINGULAR;
+  for (int i = 0; i < P->nnodes; ++i) if (ni[i]) return HODLR_ERR_SINGULAR;
+  return HODLR_OK;
+}
+
+int hodlr_workspace(hodlr_plan_t P, int32_t sys, void** ptr, int64_t* ld, int32_t* dtype) {
+  if (!P || sys < 0 || sys >= int(P->sys.size())) return HODLR_ERR_ARG;
+  if (ptr) *ptr = P->ws.p + P->sys[sys].xws;
+  if (ld) *ld = P->sys[sys].ldw;
+  if (dtype) *dtype = P->wt == F64 ? 0 : 1;
+  return HODLR_OK;
+}
+
+int hodlr_use_graph(hodlr_plan_t P, int32_t enable) {
+  if (!P) return HODLR_ERR_ARG;
+  if (enable && P->stream == nullptr) return HODLR_ERR_ARG;
+  P->use_graph = enable != 0;
+  return HODLR_OK;
+}
+
+int hodlr_profile_enable(hodlr_plan_t P, int32_t enable) {
+  if (!P) return HODLR_ERR_ARG;
+  P->profile = enable != 0;
+  return HODLR_OK;
+}
+
+int hodlr_profile_reset(hodlr_plan_t P) {
+  if (!P) return HODLR_ERR_ARG;
+  for (int i = 0; i < CAT_N; ++i) { P->prof_ms[i] = 0; P->prof_n[i] = 0; P->prof_flops[i] = 0; P->prof_bytes[i] = 0; }
+  return HODLR_OK;
+}
+
+int hodlr_profile_read(hodlr_plan_t P, double* times_ms, int32_t* launches, double* flops, double* bytes) {
+  if (!P) return HODLR_ERR_ARG;
+  for (int i = 0; i < CAT_N; ++i) {
+    if (times_ms) times_ms[i] = P->prof_ms[i];
+    if (launches) launches[i] = P->prof_n[i];
+    if (flops) flops[i] = P->prof_flops[i];
+    if (bytes) bytes[i] = P->prof_bytes[i];
+  }
+  return HODLR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
+same host-generated inputs.
+
+Tolerances (FP64 path): relative solution error and relative residual
+<= 1e-12, the bound the benchmark metric is quoted at.  The oracle's own
+rounding differs from the GPU's (different summation order inside every
+product, as the reference's nondeterministic REDOP_ADD fold does), so parity
+is tolerance based, never bitwise.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fastsolver_amd import gen, hodlr
+from tests.gpu_util import colblock_rel, host_system, oracle_solve, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _solve_fp64(spec, X, V, K, graph=False):
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_FP64)
+    if graph:
+        plan.use_graph(True)
+    plan.factor_solve()
+    plan.synchronize()
+    rc, li, ni = plan.status()
+    assert rc == 0, (li, ni)
+    return plan, ds
+
+
+def _factor_blocks(spec):
+    nl, md, xc, vc, kp = gen.tree_info(spec)
+    c, r = spec.nrhs, spec.ranks
+    blocks = [(0, c)]
+    col = c
+    for d in range(md):
+        rr = r[0] if len(r) == 1 else r[d]
+        blocks.append((col, col + rr))
+        col += rr
+    return blocks
+
+
+@pytest.mark.parametrize("case", ["single_launch", "accuracy"])
+def test_reference_circulant_cases(case):
+    """The reference's own test matrices (test/single_launch.cc, test/main.cc):
+    solution vs the Woodbury closed form, factor columns vs the oracle."""
+    spec = gen.single_launch_case() if case == "single_launch" else gen.accuracy_case()
+    print(f"case {case} n={spec.n} r={spec.ranks}", flush=True)
+    X, V, K = host_system(spec)
+    b = X[:, :spec.nrhs].copy(order="F")
+    xw = oracle.woodbury_circulant(spec.n, spec.ranks[0], spec.diag, b)
+    Xo = oracle_solve(spec, X, V, K)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    x = ds.solution()
+    assert rel(x, xw) <= TOL, rel(x, xw)
+    W = plan.workspace(0)
+    # every column block of the factor [x | A_sub^-1 U] against the oracle's
+    assert colblock_rel(W, Xo, _factor_blocks(spec)) <= TOL
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 8), (1, 1), (2, 16), (3, 32)])
+def test_random_families_fp64(cfg, scale):
+    spec = gen.config(cfg, scale=scale)[0]
+    print(f"cfg{cfg}/{scale} n={spec.n} r={spec.ranks} c={spec.nrhs}", flush=True)
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    x = ds.solution()
+    e = rel(x, Xo[:, :spec.nrhs])
+    relres, _ = plan.residual()
+    print(f"  rel err {e:.3e} relres {relres:.3e}", flush=True)
+    assert e <= TOL
+    assert relres <= TOL
+    W = plan.workspace(0)
+    assert colblock_rel(W, Xo, _factor_blocks(spec)) <= 1e-11
+
+
+def test_phases_match_fused_pass():
+    """Driving leaf / reduce / node / broadcast per height gives the fused result."""
+    spec = gen.config(1, scale=8)[0]
+    X, V, K = host_system(spec)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    x_fused = ds.solution()
+    ds2 = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    p2 = hodlr.Plan([ds2], hodlr.PREC_FP64)
+    p2.leaf_solve()
+    for h in range(1, p2.info().height + 1):
+        p2.level_reduce(h)
+        p2.level_node_solve(h)
+        p2.level_broadcast(h)
+    p2.extract_solution()
+    p2.synchronize()
+    assert np.array_equal(ds2.solution(), x_fused)
+
+
+def test_matvec_and_solve_only():
+    spec = gen.config(1, scale=4)[0]
+    X, V, K = host_system(spec)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    rng = np.random.default_rng(7)
+    z = np.asfortranarray(rng.standard_normal((spec.n, spec.nrhs)))
+    yo = oracle.hodlr_matvec(spec.as_dict(), X, V, K, z)
+    dz = hodlr.DeviceArray.from_host(z)
+    dy = hodlr.DeviceArray(spec.n, spec.nrhs)
+    plan.matvec([dz], [dy])
+    assert rel(dy.download(), yo) <= 1e-13
+    # solve-only with the stored factorization recovers z from A z
+    dyb = hodlr.DeviceArray.from_host(yo)
+    dzo = hodlr.DeviceArray(spec.n, spec.nrhs)
+    plan.solve_only([dyb], [dzo])
+    assert rel(dzo.download(), z) <= TOL
+
+
+def test_graph_replay_matches_eager():
+    spec = gen.config(1, scale=8)[0]
+    X, V, K = host_system(spec)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    x0 = ds.solution()
+    ds2 = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    p2 = hodlr.Plan([ds2], hodlr.PREC_FP64)
+    p2.use_graph(True)
+    for it in range(3):
+        p2.factor_solve()
+        p2.synchronize()
+        assert np.array_equal(ds2.solution(), x0), it
+        plan.factor_solve()   # eager work between replays
+        plan.synchronize()
+
+
+def test_fp32_factor_with_fp64_refinement():
+    spec = gen.config(1, scale=4)[0]
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_FP32)
+    rep = plan.solve(tol=TOL, maxiter=10)
+    print(f"  fp32: first relres {rep.relres_first:.3e}, {rep.iterations} sweeps, relres {rep.relres:.3e}", flush=True)
+    assert rep.converged and rep.relres <= TOL
+    assert rep.relres_first < 1e-4
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
+
+
+def test_uneven_batch_one_plan():
+    """Independent systems of different N, per-level ranks and nrhs in one plan."""
+    specs = gen.batched_uneven(6, nmin_blocks=2, nmax_blocks=12)
+    hosts = [host_system(s) for s in specs]
+    dss = [hodlr.DeviceSystem.from_host(s, *h) for s, h in zip(specs, hosts)]
+    plan = hodlr.Plan(dss, hodlr.PREC_FP64)
+    plan.factor_solve()
+    plan.synchronize()
+    for s, h, d in zip(specs, hosts, dss):
+        Xo = oracle_solve(s, *h)
+        assert rel(d.solution(), Xo[:, :s.nrhs]) <= TOL, s
+
+
+def test_odd_splits_and_per_level_ranks():
+    """N not a power of two: leaves of 62/63 rows at unequal depths' splits."""
+    spec = gen.SystemSpec(1000, 100, 3, [20, 16, 12, 9], d_scale=1 / 16)
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    assert rel(ds.solution(), Xo[:, :3]) <= TOL
+
+
+def test_nonuniform_depth_tree():
+    """threshold between N/2 and N - N/2: sibling subtrees of different depth."""
+    spec = gen.SystemSpec(1001, 250, 2, [12], d_scale=1 / 16)
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    plan, ds = _solve_fp64(spec, X, V, K)
+    assert rel(ds.solution(), Xo[:, :2]) <= TOL
+
+
+def test_singular_leaf_reports_status():
+    spec = gen.SystemSpec(512, 128, 1, [8], d_scale=1 / 16)
+    X, V, K = host_system(spec)
+    K2 = K.copy()
+    m = 128
+    K2[2 * m * m: 3 * m * m] = 0.0   # leaf 2 is all zeros: first pivot is zero
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K2)
+    plan = hodlr.Plan([ds], hodlr.PREC_FP64)
+    plan.factor_solve()
+    plan.synchronize()
+    rc, li, ni = plan.status()
+    assert rc == hodlr_err_singular()
+    assert li[2] == 1 and np.all(np.delete(li, 2) == 0)
+
+
+def hodlr_err_singular():
+    return -6
