@@ -58,6 +58,11 @@ DEVICE_SYMBOLS = {
     "hodlr_profile_category": ([_i32], ctypes.c_char_p),
     "hodlr_error_string": ([_i32], ctypes.c_char_p),
     "hodlr_version": ([], ctypes.c_char_p),
+    "hodlr_dgemm_batched": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr],
+                            ctypes.c_int),
+    "hodlr_sgemm_batched": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr],
+                            ctypes.c_int),
+    "hodlr_dgesv_batched": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
     "hodlr_device_count": ([ctypes.POINTER(_i32)], ctypes.c_int),
     "hodlr_device_name": ([ctypes.c_char_p, _i32], ctypes.c_int),
     "hodlr_device_malloc": ([ctypes.POINTER(_ptr), _i64], ctypes.c_int),

@@ -48,7 +48,8 @@ struct Tile {
   static constexpr int PB = BN + 1;
   static constexpr int WM = BM / 32;
   static constexpr int WN = BN / 32;
-  static constexpr int LDS_ELEMS = BK * (PA + PB);
+  static constexpr int STAGE = BK * (PA + PB);   // one LDS stage (A then B)
+  static constexpr int LDS_ELEMS = 2 * STAGE;    // double buffered
   typedef typename Mfma<MT>::acc_t acc_t;
   acc_t acc[WM][WN];
 
@@ -61,17 +62,19 @@ struct Tile {
         for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
   }
 
-  __device__ __forceinline__ void mma(const MT* As, const MT* Bs) {
+  // acc += As(BM x KS) * Bs(KS x BN) with pitches pa / pb (KS multiple of 4)
+  template <int KS>
+  __device__ __forceinline__ void mma(const MT* As, const MT* Bs, int pa = PA, int pb = PB) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m0 = (wave & 1) * (BM / 2), n0 = (wave >> 1) * (BN / 2);
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
+    for (int kk = 0; kk < KS; kk += 4) {
       MT a[WM], b[WN];
 #pragma unroll
-      for (int i = 0; i < WM; ++i) a[i] = As[(kk + lk) * PA + m0 + i * 16 + li];
+      for (int i = 0; i < WM; ++i) a[i] = As[(kk + lk) * pa + m0 + i * 16 + li];
 #pragma unroll
-      for (int j = 0; j < WN; ++j) b[j] = Bs[(kk + lk) * PB + n0 + j * 16 + li];
+      for (int j = 0; j < WN; ++j) b[j] = Bs[(kk + lk) * pb + n0 + j * 16 + li];
 #pragma unroll
       for (int i = 0; i < WM; ++i)
 #pragma unroll
@@ -93,49 +96,93 @@ struct Tile {
   }
 };
 
-// Load an A tile (rows [r0, r0+BM) x k-columns [k0, k0+BK)) into As[k][m].
-// TRANS = false: A(m, k) at A[m + k*lda];  TRANS = true: A(m, k) at A[k + m*lda].
-// Out-of-range entries (m >= mlim or k >= klim) are zero.
-template <class MT, int BM, int BK, bool TRANS, class TA>
-__device__ __forceinline__ void load_a(MT* As, const TA* A, int64_t lda, int r0, int mlim, int k0, int klim) {
-  constexpr int PA = BM + 1;
-  if (!TRANS) {
+// Register staging of one A tile (rows [r0, r0+BM) x k [k0, k0+BK)) and one B
+// tile (k [k0, k0+BK) x cols [c0, c0+BN)); out-of-range entries are zero.
+//   A: TRANS = false -> A(m, k) at A[m + k*lda];  TRANS = true -> A[k + m*lda]
+//   B: B(k, n) at B[k + n*ldb]
+template <class MT, int BM, int BN, int BK, bool TRANS>
+struct Stager {
+  static constexpr int NA = (BM * BK) / kThreads, NB = (BN * BK) / kThreads;
+  static constexpr int PA = BM + 1, PB = BN + 1;
+  MT ra[NA], rb[NB];
+
+  template <class TA>
+  __device__ __forceinline__ void load_a(const TA* A, int64_t lda, int r0, int mlim, int k0, int klim) {
 #pragma unroll
-    for (int s = 0; s < (BM * BK) / kThreads; ++s) {
-      int e = threadIdx.x + s * kThreads;
-      int m = e % BM, k = e / BM;
-      int gm = r0 + m, gk = k0 + k;
-      MT v = 0;
-      if (gm < mlim && gk < klim) v = cvt<MT>(A[gm + int64_t(gk) * lda]);
-      As[k * PA + m] = v;
+    for (int s = 0; s < NA; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
+      const int gm = r0 + m, gk = k0 + k;
+      ra[s] = (gm < mlim && gk < klim) ? MT(TRANS ? A[gk + int64_t(gm) * lda] : A[gm + int64_t(gk) * lda]) : MT(0);
     }
-  } else {
+  }
+  template <class TB>
+  __device__ __forceinline__ void load_b(const TB* B, int64_t ldb, int k0, int klim, int c0, int nlim) {
 #pragma unroll
-    for (int s = 0; s < (BM * BK) / kThreads; ++s) {
-      int e = threadIdx.x + s * kThreads;
-      int k = e % BK, m = e / BK;
-      int gm = r0 + m, gk = k0 + k;
-      MT v = 0;
-      if (gm < mlim && gk < klim) v = cvt<MT>(A[gk + int64_t(gm) * lda]);
-      As[k * PA + m] = v;
+    for (int s = 0; s < NB; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int k = e % BK, n = e / BK;
+      const int gk = k0 + k, gn = c0 + n;
+      rb[s] = (gk < klim && gn < nlim) ? MT(B[gk + int64_t(gn) * ldb]) : MT(0);
     }
+  }
+  __device__ __forceinline__ void store(MT* As, MT* Bs) const {
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
+      As[k * PA + m] = ra[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      Bs[(e % BK) * PB + e / BK] = rb[s];
+    }
+  }
+};
+
+// acc += op(A)[r0.., kb..ke) * B[kb..ke), c0..) with register prefetch and two
+// LDS stages: global loads of step t+1 are in flight while step t computes.
+template <class MT, int BM, int BN, int BK, bool TRANS, class TA, class TB>
+__device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, const TA* A, int64_t lda, int r0,
+                                          int mlim, const TB* B, int64_t ldb, int c0, int nlim, int kb, int ke) {
+  typedef Tile<MT, BM, BN, BK> T;
+  if (kb >= ke) return;
+  Stager<MT, BM, BN, BK, TRANS> st;
+  st.load_a(A, lda, r0, mlim, kb, ke);
+  st.load_b(B, ldb, kb, ke, c0, nlim);
+  st.store(lds, lds + BK * T::PA);
+  __syncthreads();
+  int stage = 0;
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    const bool more = k0 + BK < ke;
+    if (more) {
+      st.load_a(A, lda, r0, mlim, k0 + BK, ke);
+      st.load_b(B, ldb, k0 + BK, ke, c0, nlim);
+    }
+    MT* cur = lds + stage * T::STAGE;
+    t.template mma<BK>(cur, cur + BK * T::PA);
+    if (more) {
+      MT* nxt = lds + (stage ^ 1) * T::STAGE;
+      st.store(nxt, nxt + BK * T::PA);
+    }
+    __syncthreads();
+    stage ^= 1;
   }
 }
 
-// Load a B tile (k-rows [k0, k0+BK) x columns [c0, c0+BN)) into Bs[k][n];
-// B(k, n) at B[k + n*ldb].
-template <class MT, int BN, int BK, class TB>
-__device__ __forceinline__ void load_b(MT* Bs, const TB* B, int64_t ldb, int k0, int klim, int c0, int nlim) {
-  constexpr int PB = BN + 1;
-#pragma unroll
-  for (int s = 0; s < (BN * BK) / kThreads; ++s) {
-    int e = threadIdx.x + s * kThreads;
-    int k = e % BK, n = e / BK;
-    int gk = k0 + k, gn = c0 + n;
-    MT v = 0;
-    if (gk < klim && gn < nlim) v = cvt<MT>(B[gk + int64_t(gn) * ldb]);
-    Bs[k * PB + n] = v;
-  }
+// Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Remap so
+// that consecutive logical tiles (which share operands) run on one XCD and
+// share its L2: logical = (bid % 8) * ceil(n/8) + bid / 8, skipping the holes
+// of the last row.
+__device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t n) {
+  if (n < 64) return bid;
+  const int64_t per = (n + 7) / 8;           // logical tiles per XCD
+  const int64_t x = bid % 8, i = bid / 8;
+  const int64_t full = n - (per - 1) * 8;    // XCDs that get `per` tiles
+  // XCDs [0, full) own per tiles, the rest per-1
+  if (x < full) return x * per + i;
+  return full * per + (x - full) * (per - 1) + i;
 }
 
 // Index of the descriptor that owns flattened tile `bid` (tile_begin ascending).

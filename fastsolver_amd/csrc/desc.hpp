@@ -41,6 +41,7 @@ struct LuDesc {
   int32_t* ipiv;      // m entries, 0-based row swapped with row i at step i
   int32_t* perm;      // m entries, row i of P*A is row perm[i] of A
   int32_t* info;      // first zero pivot (1-based), 0 if none
+  void* dinv;         // per 64-row diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 each (ld 64)
   int64_t tile_begin; // used by the row-block launch
 };
 
@@ -48,6 +49,7 @@ struct LuDesc {
 struct TrsmDesc {
   const void* LU;
   int64_t ldlu;
+  const void* dinv;     // inverted diagonal blocks of LU (LuDesc::dinv)
   const int32_t* perm;  // may be null: no row interchanges
   const void* Xin;
   int64_t ldin;

@@ -27,10 +27,8 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restri
   constexpr int BK = kGemmBK;
   typedef Tile<MT, BM, BN, BK> TileT;
   __shared__ MT lds[TileT::LDS_ELEMS];
-  MT* As = lds;
-  MT* Bs = lds + BK * TileT::PA;
 
-  const int64_t bid = blockIdx.x;
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const int p = find_desc(descs, ndesc, bid);
   const GemmDesc d = descs[p];
   const int64_t local = bid - d.tile_begin;
@@ -41,18 +39,11 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restri
   const int c0 = (rem / d.tiles_m) * BN;
   const int kb = split * d.kchunk;
   const int ke = min(d.k, kb + d.kchunk);
-  const TA* A = (const TA*)d.A;
-  const TB* B = (const TB*)d.B;
 
   TileT t;
   t.zero();
-  for (int k0 = kb; k0 < ke; k0 += BK) {
-    load_a<MT, BM, BK, TRANS>(As, A, d.lda, r0, d.m, k0, ke);
-    load_b<MT, BN, BK>(Bs, B, d.ldb, k0, ke, c0, d.n);
-    __syncthreads();
-    t.mma(As, Bs);
-    __syncthreads();
-  }
+  gemm_loop<MT, BM, BN, BK, TRANS>(t, lds, (const TA*)d.A, d.lda, r0, d.m, (const TB*)d.B, d.ldb, c0, d.n, kb, ke);
+
   const double alpha = d.alpha, beta = d.beta;
   const int m = d.m, n = d.n;
   if (d.nsplit > 1) {
@@ -116,19 +107,37 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 }
 
 // ========================================================================= LU
-// Panel of kLuNB columns starting at k0, factored as two 32-column
-// sub-panels held in registers (thread t owns rows t, t+256, ... of the
-// sub-panel).  Partial pivoting picks the first largest |a| (idamax); the
-// column below a zero pivot is left unscaled and info records it, as dgetf2.
-constexpr int kPW = 32;
+// Panel of kLuNB columns starting at k0, factored as kLuNB/PW sub-panels of PW
+// columns held in registers: 1024 threads, thread t owns row ks + t of the
+// sub-panel (blocks of up to 1024 rows).  PW = 16 for fp64 and 32 for fp32
+// keeps the register panel within the 128 VGPRs a 1024-thread group allows.
+// Partial pivoting picks the first largest |a| (idamax); the column below a
+// zero pivot is left unscaled and info records it, as dgetf2 does.
+constexpr int kPanelThreads = 1024;
+constexpr int kPanelWaves = kPanelThreads / 64;
+
+template <class T> struct PanelWidth { static constexpr int value = 16; };
+template <> struct PanelWidth<float> { static constexpr int value = 32; };
 
 template <class T>
 __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
   if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
 }
 
-template <class T, int RPT>
-__global__ __launch_bounds__(kThreads) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+template <class T>
+__device__ __forceinline__ void swap_rows_in_column(T* A, int64_t col, int r1, int r2) {
+  if (r1 != r2) {
+    T x = A[r1 + col];
+    A[r1 + col] = A[r2 + col];
+    A[r2 + col] = x;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kPanelThreads) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+  constexpr int PW = PanelWidth<T>::value;
+  constexpr int NSP = kLuNB / PW;
+  constexpr int PX = kLuNB + 1;
   const LuDesc d = descs[blockIdx.x];
   const int m = d.m;
   if (k0 >= m) return;
@@ -137,44 +146,33 @@ __global__ __launch_bounds__(kThreads) void lu_panel_kernel(const LuDesc* __rest
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nb = min(kLuNB, m - k0);
 
-  __shared__ T s_rowp[2][kPW];
-  __shared__ T s_rowj[2][kPW];
-  __shared__ T s_rv[2][4];
-  __shared__ int s_ri[2][4];
+  __shared__ T s_rowp[2][PW];
+  __shared__ T s_rowj[2][PW];
+  __shared__ T s_rv[2][kPanelWaves];
+  __shared__ int s_ri[2][kPanelWaves];
   __shared__ int s_piv[kLuNB];        // absolute pivot row of panel column jg
-  __shared__ T s_l[kPW * (kPW + 1)];  // L11 of sub-panel 0
-  __shared__ T s_u[kPW * (kPW + 1)];  // U12 block
+  __shared__ T s_l[PW * (PW + 1)];    // L11 of the current sub-panel
+  __shared__ T s_x[PW * PX];          // U12 rows of the current sub-panel
   __shared__ int s_info;
   if (tid == 0) s_info = 0;
   __syncthreads();
 
-  for (int sp = 0; sp < 2; ++sp) {
-    const int ks = k0 + kPW * sp;
+  for (int sp = 0; sp < NSP; ++sp) {
+    const int ks = k0 + PW * sp;
     if (ks >= k0 + nb) break;
-    const int w = min(kPW, k0 + nb - ks);
+    const int w = min(PW, k0 + nb - ks);
     const int rows = m - ks;
-    T a[RPT][kPW];
+    const bool mine = tid < rows;
+    T a[PW];
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int q = tid + kThreads * i;
+    for (int jj = 0; jj < PW; ++jj) a[jj] = (mine && jj < w) ? A[(ks + tid) + int64_t(ks + jj) * lda] : T(0);
 #pragma unroll
-      for (int jj = 0; jj < kPW; ++jj)
-        a[i][jj] = (q < rows && jj < w) ? A[(ks + q) + int64_t(ks + jj) * lda] : T(0);
-    }
-#pragma unroll
-    for (int j = 0; j < kPW; ++j) {
+    for (int j = 0; j < PW; ++j) {
       if (j < w) {
         const int buf = j & 1;
-        T bv = T(-1);
-        int bi = 0x7fffffff;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-          const int q = tid + kThreads * i;
-          if (q >= j && q < rows) {
-            T v = fabs(a[i][j]);
-            if (v > bv) { bv = v; bi = q; }
-          }
-        }
+        T bv = (mine && tid >= j) ? T(fabs(a[j])) : T(-1);
+        int bi = (mine && tid >= j) ? tid : 0x7fffffff;
+        if (!(bv == bv)) { bv = T(-1); bi = 0x7fffffff; }   // NaN rows never win
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
           T ov = __shfl_xor(bv, off);
@@ -186,25 +184,18 @@ __global__ __launch_bounds__(kThreads) void lu_panel_kernel(const LuDesc* __rest
         T pv = s_rv[buf][0];
         int p = s_ri[buf][0];
 #pragma unroll
-        for (int q = 1; q < 4; ++q) pivot_combine(pv, p, s_rv[buf][q], s_ri[buf][q]);
-        // a column of NaNs offers no candidate: keep row j (no interchange)
-        if (p >= rows) p = j;
-        // publish the pivot row and row j
-        const int ip = p / kThreads;
-        if (tid == p % kThreads) {
+        for (int q = 1; q < kPanelWaves; ++q) pivot_combine(pv, p, s_rv[buf][q], s_ri[buf][q]);
+        if (p >= rows) p = j;   // no candidate (all NaN): keep row j
+        if (tid == p) {
 #pragma unroll
-          for (int i = 0; i < RPT; ++i)
-            if (i == ip) {
-#pragma unroll
-              for (int jj = 0; jj < kPW; ++jj) s_rowp[buf][jj] = a[i][jj];
-            }
+          for (int jj = 0; jj < PW; ++jj) s_rowp[buf][jj] = a[jj];
         }
         if (tid == j) {
 #pragma unroll
-          for (int jj = 0; jj < kPW; ++jj) s_rowj[buf][jj] = a[0][jj];
+          for (int jj = 0; jj < PW; ++jj) s_rowj[buf][jj] = a[jj];
         }
         if (tid == 0) {
-          s_piv[kPW * sp + j] = ks + p;
+          s_piv[PW * sp + j] = ks + p;
           d.ipiv[ks + j] = ks + p;
           if (!(pv > T(0)) && s_info == 0) s_info = ks + j + 1;
         }
@@ -212,122 +203,80 @@ __global__ __launch_bounds__(kThreads) void lu_panel_kernel(const LuDesc* __rest
         if (p != j) {
           if (tid == j) {
 #pragma unroll
-            for (int jj = 0; jj < kPW; ++jj) a[0][jj] = s_rowp[buf][jj];
-          }
-          if (tid == p % kThreads) {
+            for (int jj = 0; jj < PW; ++jj) a[jj] = s_rowp[buf][jj];
+          } else if (tid == p) {
 #pragma unroll
-            for (int i = 0; i < RPT; ++i)
-              if (i == ip) {
-#pragma unroll
-                for (int jj = 0; jj < kPW; ++jj) a[i][jj] = s_rowj[buf][jj];
-              }
+            for (int jj = 0; jj < PW; ++jj) a[jj] = s_rowj[buf][jj];
           }
         }
         const T piv = s_rowp[buf][j];
-        const T rinv = piv != T(0) ? T(1) / piv : T(0);
+        if (mine && tid > j) {
+          const T l = piv != T(0) ? a[j] / piv : a[j];
+          a[j] = l;
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-          const int q = tid + kThreads * i;
-          if (q > j && q < rows) {
-            const T l = piv != T(0) ? a[i][j] * rinv : a[i][j];
-            a[i][j] = l;
-#pragma unroll
-            for (int jj = j + 1; jj < kPW; ++jj) a[i][jj] -= l * s_rowp[buf][jj];
-          }
+          for (int jj = j + 1; jj < PW; ++jj) a[jj] -= l * s_rowp[buf][jj];
         }
       }
     }
-    // write the factored sub-panel back
+    if (mine) {
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int q = tid + kThreads * i;
-      if (q < rows) {
+      for (int jj = 0; jj < PW; ++jj)
+        if (jj < w) A[(ks + tid) + int64_t(ks + jj) * lda] = a[jj];
+    }
+    if (tid < PW) {
 #pragma unroll
-        for (int jj = 0; jj < kPW; ++jj)
-          if (jj < w) A[(ks + q) + int64_t(ks + jj) * lda] = a[i][jj];
-      }
+      for (int jj = 0; jj < PW; ++jj) s_l[tid * (PW + 1) + jj] = a[jj];
     }
     __syncthreads();
-    if (sp == 0 && nb > kPW) {
-      const int nc2 = nb - kPW;   // columns of sub-panel 1
-      // sub-panel 0 interchanges on sub-panel 1 columns
-      if (tid < nc2) {
-        const int64_t col = int64_t(ks + kPW + tid) * lda;
-        for (int j = 0; j < w; ++j) {
-          const int pr = s_piv[j];
-          if (pr != ks + j) {
-            T x = A[(ks + j) + col];
-            A[(ks + j) + col] = A[pr + col];
-            A[pr + col] = x;
-          }
-        }
-      }
-      if (tid < kPW) {
-#pragma unroll
-        for (int jj = 0; jj < kPW; ++jj) s_l[tid * (kPW + 1) + jj] = a[0][jj];
-      }
-      __syncthreads();
-      for (int e = tid; e < kPW * nc2; e += kThreads) {
-        const int r = e % kPW, c = e / kPW;
-        s_u[r * (kPW + 1) + c] = A[(ks + r) + int64_t(ks + kPW + c) * lda];
+    // this sub-panel's interchanges on the other panel columns, in order
+    const int nleft = ks - k0, nright = k0 + nb - (ks + PW);
+    if (tid < nleft + max(nright, 0)) {
+      const int col = tid < nleft ? k0 + tid : ks + PW + (tid - nleft);
+      const int64_t cb = int64_t(col) * lda;
+      for (int j = 0; j < w; ++j) swap_rows_in_column(A, cb, ks + j, s_piv[PW * sp + j]);
+    }
+    __syncthreads();
+    if (nright > 0) {
+      for (int e = tid; e < PW * nright; e += kPanelThreads) {
+        const int r = e % PW, c = e / PW;
+        s_x[r * PX + c] = A[(ks + r) + int64_t(ks + PW + c) * lda];
       }
       __syncthreads();
       // U12 = L11^-1 A12 (unit lower), one thread per column
-      if (tid < nc2) {
-        for (int i = 1; i < kPW; ++i) {
-          T x = s_u[i * (kPW + 1) + tid];
-          for (int k = 0; k < i; ++k) x -= s_l[i * (kPW + 1) + k] * s_u[k * (kPW + 1) + tid];
-          s_u[i * (kPW + 1) + tid] = x;
+      if (tid < nright) {
+        for (int i = 1; i < PW; ++i) {
+          T x = s_x[i * PX + tid];
+          for (int k = 0; k < i; ++k) x -= s_l[i * (PW + 1) + k] * s_x[k * PX + tid];
+          s_x[i * PX + tid] = x;
         }
       }
       __syncthreads();
-      for (int e = tid; e < kPW * nc2; e += kThreads) {
-        const int r = e % kPW, c = e / kPW;
-        A[(ks + r) + int64_t(ks + kPW + c) * lda] = s_u[r * (kPW + 1) + c];
+      for (int e = tid; e < PW * nright; e += kPanelThreads) {
+        const int r = e % PW, c = e / PW;
+        A[(ks + r) + int64_t(ks + PW + c) * lda] = s_x[r * PX + c];
       }
-      // A22 -= L21 U12 on the sub-panel 1 columns, L21 rows from registers
+      // A22 -= L21 U12 on the remaining panel columns, L21 rows from registers
+      if (mine && tid >= PW) {
+        for (int c = 0; c < nright; ++c) {
+          T x = A[(ks + tid) + int64_t(ks + PW + c) * lda];
 #pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int q = tid + kThreads * i;
-        if (q >= kPW && q < rows) {
-          for (int c = 0; c < nc2; ++c) {
-            T x = A[(ks + q) + int64_t(ks + kPW + c) * lda];
-#pragma unroll
-            for (int k = 0; k < kPW; ++k) x -= a[i][k] * s_u[k * (kPW + 1) + c];
-            A[(ks + q) + int64_t(ks + kPW + c) * lda] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (sp == 1) {
-      // sub-panel 1 interchanges on sub-panel 0 columns
-      if (tid < kPW) {
-        const int64_t col = int64_t(k0 + tid) * lda;
-        for (int j = 0; j < w; ++j) {
-          const int pr = s_piv[kPW + j];
-          if (pr != ks + j) {
-            T x = A[(ks + j) + col];
-            A[(ks + j) + col] = A[pr + col];
-            A[pr + col] = x;
-          }
+          for (int k = 0; k < PW; ++k) x -= a[k] * s_x[k * PX + c];
+          A[(ks + tid) + int64_t(ks + PW + c) * lda] = x;
         }
       }
       __syncthreads();
     }
   }
-  // panel interchanges on every column outside the panel (dlaswp)
-  const int nout = m - nb;
-  for (int c = tid; c < nout; c += kThreads) {
-    const int col = c < k0 ? c : c + nb;
-    const int64_t cb = int64_t(col) * lda;
-    for (int j = 0; j < nb; ++j) {
-      const int pr = s_piv[j];
-      if (pr != k0 + j) {
-        T x = A[(k0 + j) + cb];
-        A[(k0 + j) + cb] = A[pr + cb];
-        A[pr + cb] = x;
-      }
+  // panel interchanges on every column outside the panel (dlaswp); nothing
+  // to do when the panel kept its rows
+  bool any = false;
+  for (int j = 0; j < nb; ++j) any |= s_piv[j] != k0 + j;
+  if (any) {
+    const int nout = m - nb;
+    for (int c = tid; c < nout; c += kPanelThreads) {
+      const int col = c < k0 ? c : c + nb;
+      const int64_t cb = int64_t(col) * lda;
+      for (int j = 0; j < nb; ++j) swap_rows_in_column(A, cb, k0 + j, s_piv[j]);
     }
   }
   if (tid == 0 && s_info != 0 && *d.info == 0) *d.info = s_info;
@@ -409,108 +358,122 @@ __global__ void lu_init_kernel(const LuDesc* __restrict__ descs, int n) {
   if (i < n) *descs[i].info = 0;
 }
 
+// ====================================================== diagonal-block inverses
+// For every 64-row diagonal block b of an LU factor: L_bb^-1 (unit lower) and
+// U_bb^-1 (upper), zero outside the block's nr x nr corner.  Threads 0..63
+// invert L by columns, threads 64..127 invert U; each column is one forward /
+// backward substitution against a unit vector.
+constexpr int kDB = 64;
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restrict__ descs) {
+  const LuDesc d = descs[blockIdx.y];
+  const int b = blockIdx.x;
+  const int r0 = b * kDB;
+  if (r0 >= d.m) return;
+  const int nr = min(kDB, d.m - r0);
+  const T* A = (const T*)d.A;
+  constexpr int P = kDB + 1;
+  __shared__ T s_a[kDB * P];   // the diagonal block, s_a[i*P + k] = A(r0+i, r0+k)
+  __shared__ T s_l[kDB * P];   // L^-1, s_l[i*P + j]
+  __shared__ T s_w[kDB * P];   // U^-1
+  for (int e = threadIdx.x; e < kDB * kDB; e += kThreads) {
+    const int i = e % kDB, k = e / kDB;
+    s_a[i * P + k] = (i < nr && k < nr) ? A[(r0 + i) + int64_t(r0 + k) * d.lda] : T(0);
+    s_l[i * P + k] = T(0);
+    s_w[i * P + k] = T(0);
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  if (tid < nr) {
+    const int j = tid;
+    s_l[j * P + j] = T(1);
+    for (int i = j + 1; i < nr; ++i) {
+      T x = 0;
+      for (int k = j; k < i; ++k) x -= s_a[i * P + k] * s_l[k * P + j];
+      s_l[i * P + j] = x;
+    }
+  } else if (tid >= kDB && tid < kDB + nr) {
+    const int j = tid - kDB;
+    s_w[j * P + j] = T(1) / s_a[j * P + j];
+    for (int i = j - 1; i >= 0; --i) {
+      T x = 0;
+      for (int k = i + 1; k <= j; ++k) x -= s_a[i * P + k] * s_w[k * P + j];
+      s_w[i * P + j] = x / s_a[i * P + i];
+    }
+  }
+  __syncthreads();
+  T* out = (T*)d.dinv + int64_t(b) * 2 * kDB * kDB;
+  for (int e = threadIdx.x; e < kDB * kDB; e += kThreads) {
+    const int i = e % kDB, j = e / kDB;
+    out[e] = s_l[i * P + j];
+    out[kDB * kDB + e] = s_w[i * P + j];
+  }
+}
+
 // ======================================================================= TRSM
-// One workgroup per (block, 64-column panel): forward elimination with L
-// (unit lower) on P*X_in, then back substitution with U.  Each 64-row step
-// first applies the already solved rows with an MFMA product, then solves the
-// 64 x 64 diagonal block in LDS, one thread per column.
+// One workgroup per (block, 64-column panel): Y = L^-1 P X_in, then
+// X_out = U^-1 Y, 64 rows at a time.  Each step applies the rows already
+// solved with a prefetching MFMA product, then multiplies by the inverted
+// diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
+// serial substitution.
 template <class TLU, class TIN, class TOUT>
 __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
   typedef TLU MT;
   constexpr int BK = kGemmBK, RB = kTrsmRB, CB = kTrsmCB, P = RB + 1;
+  static_assert(RB == kDB, "diagonal blocks and TRSM steps share the 64-row blocking");
   typedef Tile<MT, RB, CB, BK> TileT;
   constexpr int UNION = (TileT::LDS_ELEMS > RB * P) ? TileT::LDS_ELEMS : RB * P;
-  __shared__ MT s_u[UNION];   // GEMM staging, then the diagonal block
-  __shared__ MT s_t[RB * P];  // right-hand-side block, s_t[r*P + c]
-  MT* As = s_u;
-  MT* Bs = s_u + BK * TileT::PA;
+  __shared__ MT s_u[UNION];   // MFMA stages, then the inverted diagonal block (As layout)
+  __shared__ MT s_t[RB * P];  // right-hand-side block, s_t[r*P + c] (Bs layout)
 
-  const int64_t bid = blockIdx.x;
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const TrsmDesc d = descs[find_desc(descs, ndesc, bid)];
   const int c0 = int(bid - d.tile_begin) * CB;
   const int m = d.m, ncols = d.ncols;
   const int nc = min(CB, ncols - c0);
   const TLU* LU = (const TLU*)d.LU;
+  const TLU* DI = (const TLU*)d.dinv;
   const TIN* Xin = (const TIN*)d.Xin;
   TOUT* Xout = (TOUT*)d.Xout;
   const int64_t ldlu = d.ldlu, ldin = d.ldin, ldout = d.ldout;
   const int nblk = (m + RB - 1) / RB;
   TileT t;
 
-  // ---- forward: Y = L^-1 P X_in
-  for (int b = 0; b < nblk; ++b) {
-    const int r0 = b * RB, nr = min(RB, m - r0);
-    t.zero();
-    for (int k0 = 0; k0 < r0; k0 += BK) {
-      load_a<MT, RB, BK, false>(As, LU, ldlu, r0, m, k0, r0);
-      load_b<MT, CB, BK>(Bs, Xout, ldout, k0, r0, c0, ncols);
-      __syncthreads();
-      t.mma(As, Bs);
-      __syncthreads();
-    }
-    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-      MT x = 0;
-      if (r < nr && c < nc) {
-        const int src = d.perm ? d.perm[r0 + r] : r0 + r;
-        x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
+  for (int pass = 0; pass < 2; ++pass) {        // 0: forward with L, 1: backward with U
+    for (int s = 0; s < nblk; ++s) {
+      const int b = pass == 0 ? s : nblk - 1 - s;
+      const int r0 = b * RB, nr = min(RB, m - r0);
+      t.zero();
+      if (pass == 0)
+        gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
+      else
+        gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
+      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+        MT x = 0;
+        if (r < nr && c < nc) {
+          if (pass == 0) {
+            const int src = d.perm ? d.perm[r0 + r] : r0 + r;
+            x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
+          } else {
+            x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]) - v;
+          }
+        }
+        s_t[r * P + c] = x;
+      });
+      const TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
+      for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
+        const int i = e % RB, k = e / RB;
+        s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
       }
-      s_t[r * P + c] = x;
-    });
-    for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
-      const int r = e % RB, k = e / RB;
-      s_u[r * P + k] = (r < nr && k < nr) ? MT(LU[(r0 + r) + int64_t(r0 + k) * ldlu]) : MT(0);
-    }
-    __syncthreads();
-    if (threadIdx.x < nc) {
-      const int c = threadIdx.x;
-      for (int i = 1; i < nr; ++i) {
-        MT x = s_t[i * P + c];
-        for (int k = 0; k < i; ++k) x -= s_u[i * P + k] * s_t[k * P + c];
-        s_t[i * P + c] = x;
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < RB * nc; e += kThreads) {
-      const int r = e % RB, c = e / RB;
-      if (r < nr) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * P + c]);
-    }
-    __syncthreads();
-  }
-  // ---- backward: X = U^-1 Y
-  for (int b = nblk - 1; b >= 0; --b) {
-    const int r0 = b * RB, nr = min(RB, m - r0);
-    t.zero();
-    for (int k0 = r0 + RB; k0 < m; k0 += BK) {
-      load_a<MT, RB, BK, false>(As, LU, ldlu, r0, r0 + nr, k0, m);
-      load_b<MT, CB, BK>(Bs, Xout, ldout, k0, m, c0, ncols);
       __syncthreads();
-      t.mma(As, Bs);
+      t.zero();
+      t.template mma<RB>(s_u, s_t, P, P);
+      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(v);
+      });
       __syncthreads();
     }
-    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-      MT x = 0;
-      if (r < nr && c < nc) x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]) - v;
-      s_t[r * P + c] = x;
-    });
-    for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
-      const int r = e % RB, k = e / RB;
-      s_u[r * P + k] = (r < nr && k < nr) ? MT(LU[(r0 + r) + int64_t(r0 + k) * ldlu]) : MT(0);
-    }
-    __syncthreads();
-    if (threadIdx.x < nc) {
-      const int c = threadIdx.x;
-      for (int i = nr - 1; i >= 0; --i) {
-        MT x = s_t[i * P + c];
-        for (int k = i + 1; k < nr; ++k) x -= s_u[i * P + k] * s_t[k * P + c];
-        s_t[i * P + c] = x / s_u[i * P + i];
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < RB * nc; e += kThreads) {
-      const int r = e % RB, c = e / RB;
-      if (r < nr) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * P + c]);
-    }
-    __syncthreads();
   }
 }
 
@@ -546,25 +509,39 @@ namespace {
 inline unsigned grid1(int64_t tiles) { return unsigned(tiles); }
 }
 
-hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, const GemmDesc* d, int n, int64_t tiles,
-                       hipStream_t s) {
+hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int tile, const GemmDesc* d, int n,
+                       int64_t tiles, hipStream_t s) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
-  constexpr int BM = kGemmBM, BN = kGemmBN;
+#define HODLR_GEMM(R, A, B, C, M, T, BM, BN) \
+  hipLaunchKernelGGL((gemm_kernel<R, A, B, C, M, T, BM, BN>), g, b, 0, s, d, n)
+  const bool big = tile == 1;
   if (ta == F64 && tb == F64 && tc == F64 && mt == F64) {
-    if (trans) hipLaunchKernelGGL((gemm_kernel<1, double, double, double, double, true, BM, BN>), g, b, 0, s, d, n);
-    else if (role == 2) hipLaunchKernelGGL((gemm_kernel<2, double, double, double, double, false, BM, BN>), g, b, 0, s, d, n);
-    else if (role == 3) hipLaunchKernelGGL((gemm_kernel<3, double, double, double, double, false, BM, BN>), g, b, 0, s, d, n);
-    else hipLaunchKernelGGL((gemm_kernel<0, double, double, double, double, false, BM, BN>), g, b, 0, s, d, n);
+    if (trans) {
+      if (big) HODLR_GEMM(1, double, double, double, double, true, 128, 128);
+      else HODLR_GEMM(1, double, double, double, double, true, 64, 64);
+    } else if (role == 2) {
+      if (big) HODLR_GEMM(2, double, double, double, double, false, 128, 128);
+      else HODLR_GEMM(2, double, double, double, double, false, 64, 64);
+    } else if (role == 3) {
+      if (big) HODLR_GEMM(3, double, double, double, double, false, 128, 128);
+      else HODLR_GEMM(3, double, double, double, double, false, 64, 64);
+    } else {
+      HODLR_GEMM(0, double, double, double, double, false, 64, 64);
+    }
   } else if (ta == F64 && tb == F32 && tc == F64 && mt == F32 && trans) {
-    hipLaunchKernelGGL((gemm_kernel<1, double, float, double, float, true, BM, BN>), g, b, 0, s, d, n);
+    if (big) HODLR_GEMM(1, double, float, double, float, true, 128, 128);
+    else HODLR_GEMM(1, double, float, double, float, true, 64, 64);
   } else if (ta == F32 && tb == F64 && tc == F32 && mt == F32 && !trans) {
-    hipLaunchKernelGGL((gemm_kernel<2, float, double, float, float, false, BM, BN>), g, b, 0, s, d, n);
+    if (big) HODLR_GEMM(2, float, double, float, float, false, 128, 128);
+    else HODLR_GEMM(2, float, double, float, float, false, 64, 64);
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
-    hipLaunchKernelGGL((gemm_kernel<3, float, float, float, float, false, BM, BN>), g, b, 0, s, d, n);
+    if (big) HODLR_GEMM(3, float, float, float, float, false, 128, 128);
+    else HODLR_GEMM(3, float, float, float, float, false, 64, 64);
   } else {
     return hipErrorInvalidValue;
   }
+#undef HODLR_GEMM
   return hipGetLastError();
 }
 
@@ -587,18 +564,18 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
 
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int rpt = (rows_max + kThreads - 1) / kThreads;
-  dim3 g(n), b(kThreads);
-  if (rpt > 4) return hipErrorInvalidValue;
-  if (t == F64) {
-    if (rpt <= 1) hipLaunchKernelGGL((lu_panel_kernel<double, 1>), g, b, 0, s, d, k0);
-    else if (rpt == 2) hipLaunchKernelGGL((lu_panel_kernel<double, 2>), g, b, 0, s, d, k0);
-    else hipLaunchKernelGGL((lu_panel_kernel<double, 4>), g, b, 0, s, d, k0);
-  } else {
-    if (rpt <= 1) hipLaunchKernelGGL((lu_panel_kernel<float, 1>), g, b, 0, s, d, k0);
-    else if (rpt == 2) hipLaunchKernelGGL((lu_panel_kernel<float, 2>), g, b, 0, s, d, k0);
-    else hipLaunchKernelGGL((lu_panel_kernel<float, 4>), g, b, 0, s, d, k0);
-  }
+  if (rows_max > kPanelThreads) return hipErrorInvalidValue;
+  dim3 g(n), b(kPanelThreads);
+  if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double>), g, b, 0, s, d, k0);
+  else hipLaunchKernelGGL((lu_panel_kernel<float>), g, b, 0, s, d, k0);
+  return hipGetLastError();
+}
+
+hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  dim3 g((mmax + kDB - 1) / kDB, n), b(kThreads);
+  if (t == F64) hipLaunchKernelGGL((lu_dinv_kernel<double>), g, b, 0, s, d);
+  else hipLaunchKernelGGL((lu_dinv_kernel<float>), g, b, 0, s, d);
   return hipGetLastError();
 }
 

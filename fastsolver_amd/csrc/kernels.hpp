@@ -17,13 +17,15 @@ constexpr int64_t kNormChunk = 65536;
 
 // Supported (A, B, C, compute) type tuples:
 //   (F64,F64,F64,F64) N and T;  (F64,F32,F64,F32) T;  (F32,F64,F32,F32) N;  (F32,F32,F32,F32) N.
-hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, const GemmDesc* d, int n, int64_t tiles,
-                       hipStream_t s);
+// tile 0: 64 x 64 workgroup tiles, 1: 128 x 128 (roles 1-3 only)
+hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int tile, const GemmDesc* d, int n,
+                       int64_t tiles, hipStream_t s);
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s);  // double partials
 hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s);
 hipError_t launch_lu_rowblock(DT t, const LuDesc* d, int n, int k0, int mmax, hipStream_t s);
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
+hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s);  // inverted 64x64 diagonal blocks
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
 // (TLU, TIN, TOUT) in {(F64,F64,F64), (F32,F64,F32), (F32,F32,F32)}; compute type = TLU
 hipError_t launch_trsm(DT tlu, DT tin, DT tout, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s);

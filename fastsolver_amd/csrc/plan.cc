@@ -46,7 +46,7 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
                                 "copy", "matvec", "norm", "other"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
-enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_ROW, K_LU_PERM, K_TRSM, K_NORM, K_NORM_FIN };
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_ROW, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN };
 
 struct Launch {
   Kind kind;
@@ -54,6 +54,7 @@ struct Launch {
   DT t[4];
   bool trans = false;
   int role = 0;
+  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128
   size_t off = 0;        // byte offset of the first descriptor in the blob
   int count = 0;
   int64_t tiles = 0;
@@ -80,11 +81,12 @@ struct SysState {
   Tree tree;
   std::vector<int32_t> ranks;
   int64_t ldw = 0;                 // leading dimension of the workspaces (= N)
-  size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0;
+  size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0, ldinv = 0;
   std::vector<size_t> lu_off;      // per leaf (elements)
+  std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
   std::vector<size_t> z_off;       // per leaf (doubles)
   std::vector<size_t> vtd, vtu, wmv;   // per node (doubles): reduce outputs
-  std::vector<size_t> sbuf, eta, spiv, sperm;  // per interior node
+  std::vector<size_t> sbuf, eta, spiv, sperm, sdinv;  // per interior node
   std::vector<int> sinfo;          // per interior node: index into node info array
   std::vector<size_t> kofs;        // per leaf (elements of K)
   size_t bnorm = 0, rnorm = 0;     // doubles, c each
@@ -117,12 +119,15 @@ struct hodlr_plan_s {
   // hip graph
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
-  // profiling
-  bool profile = false;
+  // profiling: event pairs around the launches of the selected categories,
+  // recorded asynchronously and harvested by hodlr_profile_read
+  uint32_t prof_mask = 0;
   double prof_ms[CAT_N] = {0};
   int32_t prof_n[CAT_N] = {0};
   double prof_flops[CAT_N] = {0}, prof_bytes[CAT_N] = {0};
-  std::vector<hipEvent_t> ev;
+  struct Pending { int cat; hipEvent_t a, b; double flops, bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
   // info
   hodlr_plan_info_t info{};
   bool factored = false;
@@ -154,13 +159,29 @@ struct Builder {
     int64_t ldcout; int m, n, k; double alpha, beta;
   };
   // split: desired minimum number of workgroups; 0 = never split
+  // min_wg: split the K range of beta = 0 problems into chunks of equal row
+  // count until the batch has about min_wg workgroups (0 = never split).
   void gemm(Proc& pr, Cat cat, DT ta, DT tb, DT tc, DT mt, bool trans, int role, const std::vector<G>& gs,
             int64_t min_wg = 0) {
     std::vector<GemmDesc> d;
     std::vector<SumDesc> sums;
-    int64_t base_tiles = 0;
+    // tile size: 128 x 128 for the big roles when the problems are large enough
+    // and the batch still fills the GPU
+    int tile = 0;
+    {
+      double sm = 0, sn = 0;
+      int cnt = 0;
+      int64_t big_tiles = 0;
+      for (const G& g : gs)
+        if (g.m > 0 && g.n > 0) { sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128); }
+      if (role >= 1 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0)) tile = 1;
+    }
+    const int TM = tile ? 128 : kGemmBM, TN = tile ? 128 : kGemmBN;
+    double work = 0;   // tile-rows of K work of the splittable problems
     for (const G& g : gs)
-      if (g.m > 0 && g.n > 0) base_tiles += cdiv(g.m, kGemmBM) * cdiv(g.n, kGemmBN);
+      if (g.m > 0 && g.n > 0) work += double(cdiv(g.m, TM) * cdiv(g.n, TN)) * std::max(g.k, 1);
+    const int64_t kMinChunk = 1024;
+    const int64_t chunk = min_wg > 0 ? std::max<int64_t>(kMinChunk, int64_t(work / double(min_wg))) : 0;
     int64_t tiles = 0;
     size_t part = 0;  // doubles of split-K partials
     double flops = 0, bytes = 0;
@@ -170,15 +191,11 @@ struct Builder {
       x.A = g.A; x.B = g.B; x.Cin = g.Cin; x.Cout = g.Cout;
       x.lda = g.lda; x.ldb = g.ldb; x.ldcin = g.ldcin; x.ldcout = g.ldcout;
       x.m = g.m; x.n = g.n; x.k = g.k;
-      x.tiles_m = int(cdiv(g.m, kGemmBM));
-      x.tiles_n = int(cdiv(g.n, kGemmBN));
+      x.tiles_m = int(cdiv(g.m, TM));
+      x.tiles_n = int(cdiv(g.n, TN));
       x.alpha = g.alpha; x.beta = g.beta;
       int nsplit = 1;
-      if (min_wg > 0 && base_tiles < min_wg && g.k >= 1024 && g.beta == 0.0 && tc == F64) {
-        int64_t want = cdiv(min_wg, base_tiles);
-        nsplit = int(std::min<int64_t>(want, g.k / 512));
-        nsplit = std::max(1, nsplit);
-      }
+      if (chunk > 0 && g.beta == 0.0 && tc == F64 && g.k >= 2 * chunk) nsplit = int(cdiv(g.k, chunk));
       int kchunk = int(cdiv(cdiv(std::max(g.k, 1), nsplit), kGemmBK) * kGemmBK);
       nsplit = int(cdiv(std::max(g.k, 1), kchunk));
       x.kchunk = kchunk;
@@ -214,7 +231,7 @@ struct Builder {
     Launch L;
     L.kind = K_GEMM; L.cat = cat;
     L.t[0] = ta; L.t[1] = tb; L.t[2] = tc; L.t[3] = mt;
-    L.trans = trans; L.role = role;
+    L.trans = trans; L.role = role; L.tile = tile;
     L.off = blob.add(d); L.count = int(d.size()); L.tiles = tiles;
     L.flops = flops; L.bytes = bytes;
     pr.l.push_back(L);
@@ -293,6 +310,9 @@ struct Builder {
     Launch Pm;
     Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
     pr.l.push_back(Pm);
+    Launch Dv;
+    Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.mmax = mmax;
+    pr.l.push_back(Dv);
   }
 
   void trsm(Proc& pr, Cat cat, DT tlu, DT tin, DT tout, std::vector<TrsmDesc> v) {
@@ -361,23 +381,28 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     S.lpiv = L.take(size_t(n) * 4);
     S.lperm = L.take(size_t(n) * 4);
     size_t lu_elems = 0, z_elems = 0, k_elems = 0;
+    size_t dinv_elems = 0;
     S.lu_off.resize(t.leaves.size());
+    S.dinv_off.resize(t.leaves.size());
     S.z_off.resize(t.leaves.size());
     S.kofs.resize(t.leaves.size());
     for (size_t j = 0; j < t.leaves.size(); ++j) {
       const TreeNode& x = t.nodes[t.leaves[j]];
       S.lu_off[j] = lu_elems;
       lu_elems += size_t(x.nrow) * x.nrow;
+      S.dinv_off[j] = dinv_elems;
+      dinv_elems += size_t(cdiv(x.nrow, 64)) * 2 * 64 * 64;
       S.z_off[j] = z_elems;
       z_elems += size_t(t.rsum[x.depth]) * c;
       S.kofs[j] = S.s.klayout == HODLR_K_STACKED ? size_t(x.row_beg) : k_elems;
       k_elems += size_t(x.nrow) * x.nrow;
     }
     S.lu = L.take(lu_elems * we);
+    S.ldinv = L.take(dinv_elems * we);
     S.zbuf = L.take(std::max<size_t>(z_elems, 1) * 8);
     const size_t nn = t.nodes.size();
     S.vtd.assign(nn, 0); S.vtu.assign(nn, 0); S.wmv.assign(nn, 0);
-    S.sbuf.assign(nn, 0); S.eta.assign(nn, 0); S.spiv.assign(nn, 0); S.sperm.assign(nn, 0);
+    S.sbuf.assign(nn, 0); S.eta.assign(nn, 0); S.spiv.assign(nn, 0); S.sperm.assign(nn, 0); S.sdinv.assign(nn, 0);
     S.sinfo.assign(nn, -1);
     for (size_t i = 1; i < nn; ++i) {
       const TreeNode& x = t.nodes[i];
@@ -396,6 +421,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
       S.eta[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
       S.spiv[i] = L.take(size_t(r) * 4);
       S.sperm[i] = L.take(size_t(r) * 4);
+      S.sdinv[i] = L.take(size_t(cdiv(r, 64)) * 2 * 64 * 64 * 8);
       S.sinfo[i] = node_count++;
     }
     S.bnorm = L.take(size_t(c) * 8);
@@ -442,9 +468,10 @@ void build_all(hodlr_plan_s* P, Builder& B) {
         l.ipiv = at<int32_t>(P, S.lpiv) + x.row_beg;
         l.perm = at<int32_t>(P, S.lperm) + x.row_beg;
         l.info = P->leaf_info + leaf_base + j;
+        l.dinv = P->ws.p + S.ldinv + S.dinv_off[j] * we;
         lus.push_back(l);
         TrsmDesc d{};
-        d.LU = l.A; d.ldlu = m; d.perm = l.perm;
+        d.LU = l.A; d.ldlu = m; d.perm = l.perm; d.dinv = l.dinv;
         d.Xin = S.s.X + x.row_beg; d.ldin = S.s.ldx;
         d.Xout = P->ws.p + S.xws + size_t(x.row_beg) * we; d.ldout = S.ldw;
         d.m = m; d.ncols = int(t.leaf_nrhs(t.leaves[j]));
@@ -521,10 +548,11 @@ void build_all(hodlr_plan_s* P, Builder& B) {
             l.ipiv = at<int32_t>(P, S.spiv[pi]);
             l.perm = at<int32_t>(P, S.sperm[pi]);
             l.info = P->node_info + S.sinfo[pi];
+            l.dinv = P->ws.p + S.sdinv[pi];
             slu.push_back(l);
           }
           TrsmDesc td{};
-          td.LU = Sm; td.ldlu = r; td.perm = at<int32_t>(P, S.sperm[pi]);
+          td.LU = Sm; td.ldlu = r; td.perm = at<int32_t>(P, S.sperm[pi]); td.dinv = P->ws.p + S.sdinv[pi];
           td.Xin = vtd[1]; td.ldin = r; td.Xout = eta0; td.ldout = r;
           td.m = r; td.ncols = nd;
           str.push_back(td);
@@ -648,7 +676,9 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
   const void* d = P->descs.p + L.off;
   hipStream_t s = P->stream;
   switch (L.kind) {
-    case K_GEMM: return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, (const GemmDesc*)d, L.count, L.tiles, s);
+    case K_GEMM:
+      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, L.tile, (const GemmDesc*)d, L.count, L.tiles, s);
+    case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.mmax, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
@@ -662,23 +692,43 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
   return hipErrorInvalidValue;
 }
 
+hipEvent_t take_event(hodlr_plan_s* P) {
+  if (!P->pool.empty()) {
+    hipEvent_t e = P->pool.back();
+    P->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int harvest(hodlr_plan_s* P) {
+  if (P->pending.empty()) return HODLR_OK;
+  if (hipStreamSynchronize(P->stream) != hipSuccess) return HODLR_ERR_HIP;
+  for (auto& q : P->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) return HODLR_ERR_HIP;
+    P->prof_ms[q.cat] += ms;
+    P->prof_n[q.cat] += 1;
+    P->prof_flops[q.cat] += q.flops;
+    P->prof_bytes[q.cat] += q.bytes;
+    P->pool.push_back(q.a);
+    P->pool.push_back(q.b);
+  }
+  P->pending.clear();
+  return HODLR_OK;
+}
+
 int run(hodlr_plan_s* P, const Proc& pr) {
   for (const Launch& L : pr.l) {
-    if (P->profile) {
-      hipEvent_t a, b;
-      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return HODLR_ERR_HIP;
-      (void)hipEventRecord(a, P->stream);
+    if (P->prof_mask & (1u << L.cat)) {
+      hipEvent_t a = take_event(P), b = take_event(P);
+      if (!a || !b) return HODLR_ERR_HIP;
+      if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
       if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
-      (void)hipEventRecord(b, P->stream);
-      if (hipEventSynchronize(b) != hipSuccess) return HODLR_ERR_HIP;
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, a, b);
-      P->prof_ms[L.cat] += ms;
-      P->prof_n[L.cat] += 1;
-      P->prof_flops[L.cat] += L.flops;
-      P->prof_bytes[L.cat] += L.bytes;
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
+      if (hipEventRecord(b, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+      P->pending.push_back({int(L.cat), a, b, L.flops, L.bytes});
     } else if (dispatch(P, L) != hipSuccess) {
       return HODLR_ERR_HIP;
     }
@@ -754,6 +804,9 @@ int check_system(const hodlr_system_t& s) {
 }
 
 void free_plan(hodlr_plan_s* P) {
+  harvest(P);
+  for (hipEvent_t e : P->pool) (void)hipEventDestroy(e);
+  P->pool.clear();
   if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
   if (P->ws.p) (void)hipFree(P->ws.p);
   if (P->descs.p) (void)hipFree(P->descs.p);
@@ -765,7 +818,7 @@ int sync_stream(hodlr_plan_s* P) {
 }
 
 int run_factor(hodlr_plan_s* P) {
-  if (P->use_graph && !P->profile) {
+  if (P->use_graph && !P->prof_mask) {
     if (!P->graph_exec) {
       hipGraph_t g;
       if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
@@ -1024,6 +1077,126 @@ int hodlr_solve_only(hodlr_plan_t P, const double* const* rhs, const int64_t* ld
   return HODLR_OK;
 }
 
+// ------------------------------------------------- standalone batched operators
+// The BLAS/LAPACK calls inside the reference's task bodies, batched: every
+// call builds its descriptors, runs on `stream` and synchronises it.
+static int run_oneoff(hodlr_plan_s& P, Builder& B, const Proc& pr) {
+  void* tmp = nullptr;
+  if (hipMalloc(&tmp, std::max<size_t>(B.blob.data.size(), 256)) != hipSuccess) return HODLR_ERR_NOMEM;
+  int rc = hipMemcpyAsync(tmp, B.blob.data.data(), B.blob.data.size(), hipMemcpyHostToDevice, P.stream) == hipSuccess
+               ? HODLR_OK : HODLR_ERR_HIP;
+  char* save = P.descs.p;
+  P.descs.p = (char*)tmp;
+  if (!rc) rc = run(&P, pr);
+  P.descs.p = save;
+  if (hipStreamSynchronize(P.stream) != hipSuccess && !rc) rc = HODLR_ERR_HIP;
+  (void)hipFree(tmp);
+  return rc;
+}
+
+static int gemm_batched(DT t, int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
+                        const double* alpha, const void* const* A, const int64_t* lda, const void* const* B,
+                        const int64_t* ldb, const double* beta, void* const* C, const int64_t* ldc, void* stream) {
+  if (batch <= 0 || !transa || !m || !n || !k || !alpha || !A || !lda || !B || !ldb || !beta || !C || !ldc)
+    return HODLR_ERR_ARG;
+  hodlr_plan_s P;
+  P.stream = (hipStream_t)stream;
+  Builder Bd{&P};
+  Proc pr;
+  const size_t es = esize(t);
+  for (int tr = 0; tr < 2; ++tr) {
+    std::vector<Builder::G> gs;
+    for (int i = 0; i < batch; ++i) {
+      const bool T = transa[i] == 't' || transa[i] == 'T';
+      if (!T && transa[i] != 'n' && transa[i] != 'N') return HODLR_ERR_ARG;
+      if (T != bool(tr) || m[i] <= 0 || n[i] <= 0) continue;
+      if (k[i] < 0 || lda[i] < 1 || ldb[i] < std::max(k[i], 1) || ldc[i] < m[i]) return HODLR_ERR_ARG;
+      if (lda[i] < (T ? std::max(k[i], 1) : m[i])) return HODLR_ERR_ARG;
+      Builder::G g{};
+      g.A = A[i]; g.lda = lda[i]; g.B = B[i]; g.ldb = ldb[i];
+      g.Cin = C[i]; g.ldcin = ldc[i]; g.Cout = C[i]; g.ldcout = ldc[i];
+      g.m = m[i]; g.n = n[i]; g.k = k[i]; g.alpha = alpha[i]; g.beta = beta[i];
+      gs.push_back(g);
+    }
+    if (gs.empty()) continue;
+    if (t == F32 && tr) return HODLR_ERR_UNSUPPORTED;   // fp32 products are NN only
+    Bd.gemm(pr, CAT_OTHER, t, t, t, t, tr == 1, tr == 1 ? 1 : (t == F32 ? 3 : 0), gs);
+  }
+  (void)es;
+  return run_oneoff(P, Bd, pr);
+}
+
+int hodlr_dgemm_batched(int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
+                        const double* alpha, const double* const* A, const int64_t* lda, const double* const* B,
+                        const int64_t* ldb, const double* beta, double* const* C, const int64_t* ldc, void* stream) {
+  return gemm_batched(F64, batch, transa, m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B, ldb, beta,
+                      (void* const*)C, ldc, stream);
+}
+
+int hodlr_sgemm_batched(int32_t batch, const int32_t* m, const int32_t* n, const int32_t* k, const double* alpha,
+                        const float* const* A, const int64_t* lda, const float* const* B, const int64_t* ldb,
+                        const double* beta, float* const* C, const int64_t* ldc, void* stream) {
+  if (batch <= 0) return HODLR_ERR_ARG;
+  std::vector<char> tn(batch, 'n');
+  return gemm_batched(F32, batch, tn.data(), m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B, ldb,
+                      beta, (void* const*)C, ldc, stream);
+}
+
+int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
+                        double* const* B, const int64_t* ldb, int32_t* info, void* stream) {
+  if (batch <= 0 || !n || !nrhs || !A || !lda || !B || !ldb) return HODLR_ERR_ARG;
+  for (int i = 0; i < batch; ++i) {
+    if (n[i] <= 0 || nrhs[i] < 0 || lda[i] < n[i] || ldb[i] < n[i]) return HODLR_ERR_ARG;
+    if (n[i] > kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  }
+  hodlr_plan_s P;
+  P.stream = (hipStream_t)stream;
+  // workspace: pivots, permutations, info, inverted diagonal blocks, a copy of B
+  Layout L;
+  std::vector<size_t> piv(batch), prm(batch), dv(batch), bc(batch);
+  for (int i = 0; i < batch; ++i) {
+    piv[i] = L.take(size_t(n[i]) * 4);
+    prm[i] = L.take(size_t(n[i]) * 4);
+    dv[i] = L.take(size_t(cdiv(n[i], 64)) * 2 * 64 * 64 * 8);
+    bc[i] = L.take(size_t(n[i]) * std::max(nrhs[i], 1) * 8);
+  }
+  size_t inf = L.take(size_t(batch) * 4);
+  if (hipMalloc((void**)&P.ws.p, L.top) != hipSuccess) return HODLR_ERR_NOMEM;
+  P.ws.bytes = L.top;
+  Builder Bd{&P};
+  Proc pr;
+  std::vector<LuDesc> lus;
+  std::vector<CopyDesc> cps;
+  std::vector<TrsmDesc> trs;
+  for (int i = 0; i < batch; ++i) {
+    LuDesc l{};
+    l.A = A[i]; l.lda = lda[i]; l.m = n[i];
+    l.ipiv = (int32_t*)(P.ws.p + piv[i]);
+    l.perm = (int32_t*)(P.ws.p + prm[i]);
+    l.info = (int32_t*)(P.ws.p + inf) + i;
+    l.dinv = P.ws.p + dv[i];
+    lus.push_back(l);
+    CopyDesc c{};
+    c.src = B[i]; c.lds = ldb[i]; c.dst = P.ws.p + bc[i]; c.ldd = n[i];
+    c.rows = n[i]; c.cols = nrhs[i]; c.alpha = 1.0;
+    cps.push_back(c);
+    TrsmDesc t{};
+    t.LU = A[i]; t.ldlu = lda[i]; t.dinv = l.dinv; t.perm = l.perm;
+    t.Xin = P.ws.p + bc[i]; t.ldin = n[i]; t.Xout = B[i]; t.ldout = ldb[i];
+    t.m = n[i]; t.ncols = nrhs[i];
+    trs.push_back(t);
+  }
+  Bd.lu(pr, CAT_OTHER, CAT_OTHER, CAT_OTHER, CAT_OTHER, F64, lus);
+  Bd.copy(pr, CAT_OTHER, F64, F64, cps);
+  Bd.trsm(pr, CAT_OTHER, F64, F64, F64, trs);
+  int rc = run_oneoff(P, Bd, pr);
+  if (!rc && info && hipMemcpy(info, P.ws.p + inf, size_t(batch) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = HODLR_ERR_HIP;
+  (void)hipFree(P.ws.p);
+  P.ws.p = nullptr;
+  return rc;
+}
+
 int hodlr_matvec(hodlr_plan_t P, const double* const* x, const int64_t* ldx, double* const* y,
                  const int64_t* ldy) {
   if (!P || !x || !ldx || !y || !ldy) return HODLR_ERR_ARG;
@@ -1161,20 +1334,23 @@ int hodlr_use_graph(hodlr_plan_t P, int32_t enable) {
   return HODLR_OK;
 }
 
-int hodlr_profile_enable(hodlr_plan_t P, int32_t enable) {
+int hodlr_profile_enable(hodlr_plan_t P, int32_t mask) {
   if (!P) return HODLR_ERR_ARG;
-  P->profile = enable != 0;
+  P->prof_mask = uint32_t(mask);
   return HODLR_OK;
 }
 
 int hodlr_profile_reset(hodlr_plan_t P) {
   if (!P) return HODLR_ERR_ARG;
+  int rc = harvest(P);
   for (int i = 0; i < CAT_N; ++i) { P->prof_ms[i] = 0; P->prof_n[i] = 0; P->prof_flops[i] = 0; P->prof_bytes[i] = 0; }
-  return HODLR_OK;
+  return rc;
 }
 
 int hodlr_profile_read(hodlr_plan_t P, double* times_ms, int32_t* launches, double* flops, double* bytes) {
   if (!P) return HODLR_ERR_ARG;
+  int rc = harvest(P);
+  if (rc) return rc;
   for (int i = 0; i < CAT_N; ++i) {
     if (times_ms) times_ms[i] = P->prof_ms[i];
     if (launches) launches[i] = P->prof_n[i];

@@ -231,8 +231,21 @@ class Plan:
     def use_graph(self, enable=True):
         check(self.lib.hodlr_use_graph(self.h, int(enable)), "hodlr_use_graph")
 
-    def profile(self, enable=True):
-        check(self.lib.hodlr_profile_enable(self.h, int(enable)))
+    CATEGORIES = None
+
+    def categories(self):
+        return [self.lib.hodlr_profile_category(i).decode() for i in range(16)]
+
+    def profile(self, categories=None):
+        """Time the launches of the named categories (None = all, [] = off)."""
+        names = self.categories()
+        if categories is None:
+            mask = (1 << 16) - 1
+        else:
+            mask = 0
+            for c in categories:
+                mask |= 1 << names.index(c)
+        check(self.lib.hodlr_profile_enable(self.h, mask))
 
     def profile_reset(self):
         check(self.lib.hodlr_profile_reset(self.h))
@@ -330,3 +343,44 @@ class FastSolver:
         self.plan = Plan([hmat.dev], self.precision)
         self.plan.factor_solve()
         return self.plan
+
+
+# ---------------------------------------------------------------------------
+# Standalone batched operators (the task bodies' BLAS/LAPACK calls)
+# ---------------------------------------------------------------------------
+def _arr(ctype, vals):
+    return (ctype * len(vals))(*vals)
+
+
+def gemm_batched(ops, dtype=np.float64, stream=None):
+    """ops: list of (transa, alpha, A, B, beta, C) with DeviceArray operands;
+    C = alpha op(A) B + beta C with op(A) = A ('n') or A^T ('t')."""
+    lib = device_lib()
+    n = len(ops)
+    tr = [o[0].encode() for o in ops]
+    m = [o[5].rows for o in ops]
+    nn = [o[5].cols for o in ops]
+    k = [(o[2].cols if o[0] in "nN" else o[2].rows) for o in ops]
+    args = (_arr(ctypes.c_int32, m), _arr(ctypes.c_int32, nn), _arr(ctypes.c_int32, k),
+            _arr(ctypes.c_double, [o[1] for o in ops]), _arr(ctypes.c_void_p, [o[2].ptr for o in ops]),
+            _arr(ctypes.c_int64, [o[2].ld for o in ops]), _arr(ctypes.c_void_p, [o[3].ptr for o in ops]),
+            _arr(ctypes.c_int64, [o[3].ld for o in ops]), _arr(ctypes.c_double, [o[4] for o in ops]),
+            _arr(ctypes.c_void_p, [o[5].ptr for o in ops]), _arr(ctypes.c_int64, [o[5].ld for o in ops]),
+            stream.handle if stream else None)
+    if dtype == np.float64:
+        check(lib.hodlr_dgemm_batched(n, ctypes.c_char_p(b"".join(tr)), *args), "hodlr_dgemm_batched")
+    else:
+        check(lib.hodlr_sgemm_batched(n, *args), "hodlr_sgemm_batched")
+
+
+def dgesv_batched(As, Bs, stream=None):
+    """Solve A_i X_i = B_i in place (A_i <- LU, B_i <- X_i); returns LAPACK info per system."""
+    lib = device_lib()
+    n = len(As)
+    info = np.zeros(n, np.int32)
+    check(lib.hodlr_dgesv_batched(n, _arr(ctypes.c_int32, [a.rows for a in As]),
+                                  _arr(ctypes.c_int32, [b.cols for b in Bs]), _arr(ctypes.c_void_p, [a.ptr for a in As]),
+                                  _arr(ctypes.c_int64, [a.ld for a in As]), _arr(ctypes.c_void_p, [b.ptr for b in Bs]),
+                                  _arr(ctypes.c_int64, [b.ld for b in Bs]), info.ctypes.data,
+                                  stream.handle if stream else None), "hodlr_dgesv_batched")
+    return info

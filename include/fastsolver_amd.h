@@ -135,6 +135,29 @@ int hodlr_solve(hodlr_plan_t plan, double tol, int32_t maxiter, hodlr_solve_repo
  * number of interior nodes, may be NULL).  Synchronises the stream. */
 int hodlr_status(hodlr_plan_t plan, int32_t* leaf_info, int32_t* node_info, int32_t* n_nodes);
 
+/* Standalone batched operators: the BLAS/LAPACK calls of the reference's
+ * task bodies on raw device pointers (host arrays of `batch` entries), for a
+ * caller that keeps its own task structure.  Each call runs on `stream`
+ * (a hipStream_t) and synchronises it.
+ *   hodlr_dgemm_batched  C = alpha op(A) B + beta C, op = 'n' or 't', FP64:
+ *                        dgemm_ of GEMM_Reduce_Task (gemm.cc:376, 't','n'),
+ *                        GEMM_Broadcast_Task (gemm.cc:473) and LUSolveTask
+ *                        (solver_tasks.cc:236,261,299)
+ *   hodlr_sgemm_batched  the same in FP32, op(A) = A
+ *   hodlr_dgesv_batched  A X = B with partial pivoting, n <= 1024: dgesv_ of
+ *                        serial_leaf_solve (solver_tasks.cc:437) and LUSolveTask
+ *                        (solver_tasks.cc:281).  A is overwritten by its LU
+ *                        factors, B by X; info[i] as LAPACK (B is not
+ *                        meaningful where info[i] > 0). */
+int hodlr_dgemm_batched(int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
+                        const double* alpha, const double* const* A, const int64_t* lda, const double* const* B,
+                        const int64_t* ldb, const double* beta, double* const* C, const int64_t* ldc, void* stream);
+int hodlr_sgemm_batched(int32_t batch, const int32_t* m, const int32_t* n, const int32_t* k, const double* alpha,
+                        const float* const* A, const int64_t* lda, const float* const* B, const int64_t* ldb,
+                        const double* beta, float* const* C, const int64_t* ldc, void* stream);
+int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
+                        double* const* B, const int64_t* ldb, int32_t* info, void* stream);
+
 /* Workspace of system `sys`: N x (c + R) factor [x | A_sub^-1 U] (the
  * reference's U region after the solve), element type 0 = double, 1 = float. */
 int hodlr_workspace(hodlr_plan_t plan, int32_t sys, void** ptr, int64_t* ld, int32_t* dtype);
@@ -143,13 +166,14 @@ int hodlr_workspace(hodlr_plan_t plan, int32_t sys, void** ptr, int64_t* ld, int
  * (needs a non-default stream). */
 int hodlr_use_graph(hodlr_plan_t plan, int32_t enable);
 
-/* Per-kernel-category device time, measured with HIP events recorded on the
- * plan's stream around every launch while enabled (graph replay disabled).
- * Categories are named by hodlr_profile_category(i), i < HODLR_PROFILE_NCAT;
- * times accumulate until hodlr_profile_reset.  Also reports the algorithmic
- * flops and bytes accumulated per category. */
+/* Per-kernel-category device time: while a category's bit is set in `mask`
+ * (bit i = category i, named by hodlr_profile_category(i), i <
+ * HODLR_PROFILE_NCAT), every launch of that category is bracketed by HIP
+ * events recorded on the plan's stream, without synchronising; read/reset
+ * harvest them.  Graph replay is off while a mask is set.  Also reports the
+ * algorithmic flops and bytes of the timed launches. */
 #define HODLR_PROFILE_NCAT 16
-int hodlr_profile_enable(hodlr_plan_t plan, int32_t enable);
+int hodlr_profile_enable(hodlr_plan_t plan, int32_t mask);
 int hodlr_profile_reset(hodlr_plan_t plan);
 int hodlr_profile_read(hodlr_plan_t plan, double* times_ms, int32_t* launches, double* flops, double* bytes);
 const char* hodlr_profile_category(int32_t i);

@@ -1,0 +1,60 @@
+"""Standalone batched operators (the task bodies' dgemm_/dgesv_) against LAPACK
+through the oracle's library and numpy, on the GPU."""
+import numpy as np
+import pytest
+
+import oracle
+from fastsolver_amd import hodlr
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dgemm_batched_nn_and_tn():
+    rng = np.random.default_rng(11)
+    ops, ref = [], []
+    for (tr, m, n, k) in [("n", 70, 33, 129), ("t", 30, 62, 60), ("n", 1, 5, 3), ("t", 128, 160, 4099)]:
+        A = np.asfortranarray(rng.standard_normal((m, k) if tr == "n" else (k, m)))
+        B = np.asfortranarray(rng.standard_normal((k, n)))
+        C = np.asfortranarray(rng.standard_normal((m, n)))
+        alpha, beta = -1.0, (1.0 if tr == "n" else 0.0)
+        ops.append((tr, alpha, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(B), beta,
+                    hodlr.DeviceArray.from_host(C)))
+        ref.append(alpha * ((A if tr == "n" else A.T) @ B) + beta * C)
+    hodlr.gemm_batched(ops)
+    for o, r in zip(ops, ref):
+        got = o[5].download()
+        assert np.abs(got - r).max() <= 1e-12 * max(1.0, np.abs(r).max()), o[0]
+
+
+def test_sgemm_batched():
+    rng = np.random.default_rng(12)
+    A = np.asfortranarray(rng.standard_normal((300, 200)).astype(np.float32))
+    B = np.asfortranarray(rng.standard_normal((200, 100)).astype(np.float32))
+    C = np.zeros((300, 100), np.float32, order="F")
+    dC = hodlr.DeviceArray.from_host(C)
+    hodlr.gemm_batched([("n", 1.0, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(B), 0.0, dC)],
+                       dtype=np.float32)
+    r = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.abs(dC.download() - r).max() <= 1e-4 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("n,nrhs", [(60, 92), (256, 181), (1000, 7), (1024, 300), (33, 1)])
+def test_dgesv_batched_vs_lapack(n, nrhs):
+    rng = np.random.default_rng(n)
+    A = np.asfortranarray(rng.standard_normal((n, n)))      # general matrix: pivoting happens
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    info_ref, X_ref = oracle.dense_dgesv(A, B)
+    dA, dB = hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(B)
+    info = hodlr.dgesv_batched([dA], [dB])
+    assert info[0] == info_ref == 0
+    X = dB.download()
+    # same LAPACK semantics: residual at rounding level and the same solution
+    assert np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)) < 1e-14
+    assert np.linalg.norm(X - X_ref) / np.linalg.norm(X_ref) < 1e-9 * np.linalg.cond(A) / 1e3
+
+
+def test_dgesv_batched_singular_info():
+    A = np.asfortranarray(np.ones((8, 8)))
+    dA, dB = hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(np.ones((8, 1)))
+    info = hodlr.dgesv_batched([dA], [dB])
+    assert info[0] == 2   # LAPACK: U(2,2) is exactly zero
