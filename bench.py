@@ -94,6 +94,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--maxiter", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--tune", type=int, default=2, help="autotuner rounds before timing (0 = off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported "
@@ -114,6 +115,9 @@ def main():
     # ------------------------------------------------------------ plain FP64
     p64 = hodlr.Plan([ds], hodlr.PREC_FP64)
     info64 = p64.info()
+    if args.tune:
+        big = p64.autotune(args.tune)
+        log(f"[bench] fp64 autotune: {big} launches on 128x128 tiles")
     if args.graph:
         p64.use_graph(True)
     t64, _ = timed_steps(p64.factor_solve, args.steps, args.warmup)
@@ -130,6 +134,9 @@ def main():
 
     # ------------------------------------------------------------ mixed + refine
     p32 = hodlr.Plan([ds], hodlr.PREC_FP32)
+    if args.tune:
+        big = p32.autotune(args.tune)
+        log(f"[bench] mixed autotune: {big} launches on 128x128 tiles")
     if args.graph:
         p32.use_graph(True)
     rep_box = {}

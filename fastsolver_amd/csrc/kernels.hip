@@ -40,31 +40,57 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restri
   const int kb = split * d.kchunk;
   const int ke = min(d.k, kb + d.kchunk);
 
+  // beta != 0: the C tile is read coalesced (column by column) into registers
+  // before the K loop so its latency hides behind the MFMA work (64 x 64 tiles)
+  constexpr int CPT = (BM * BN) / kThreads;          // C elements per thread
+  constexpr bool PREFETCH = CPT <= 16;
+  const bool use_beta = d.nsplit == 1 && d.beta != 0.0;
+  const int m = d.m, n = d.n;
+  const TC* cin = (const TC*)d.Cin;
+  const int64_t ldi = d.ldcin;
+  TC cpre[PREFETCH ? CPT : 1];
+  if (PREFETCH && use_beta) {
+#pragma unroll
+    for (int s = 0; s < (PREFETCH ? CPT : 1); ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int gr = r0 + e % BM, gc = c0 + e / BM;
+      cpre[s] = (gr < m && gc < n) ? cin[gr + gc * ldi] : TC(0);
+    }
+  }
+
   TileT t;
   t.zero();
   gemm_loop<MT, BM, BN, BK, TRANS>(t, lds, (const TA*)d.A, d.lda, r0, d.m, (const TB*)d.B, d.ldb, c0, d.n, kb, ke);
 
+  // epilogue through LDS: CH columns per pass, written column by column
+  constexpr int CH = (BM * (BN + 1) <= TileT::LDS_ELEMS) ? BN : 32;
+  constexpr int PC = BM + 1;
+  static_assert(CH * PC <= TileT::LDS_ELEMS, "epilogue chunk must fit the LDS stages");
   const double alpha = d.alpha, beta = d.beta;
-  const int m = d.m, n = d.n;
-  if (d.nsplit > 1) {
-    TC* out = (TC*)d.Cout + int64_t(split) * d.split_stride;
-    const int64_t ldo = d.ldcout;
+  TC* out = (TC*)d.Cout + (d.nsplit > 1 ? int64_t(split) * d.split_stride : 0);
+  const int64_t ldo = d.ldcout;
+  MT* cs = lds;
+#pragma unroll
+  for (int q = 0; q < BN / CH; ++q) {
     t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-      int gr = r0 + r, gc = c0 + c;
-      if (gr < m && gc < n) out[gr + gc * ldo] = TC(alpha * double(v));
+      if (c / CH == q) cs[(c - q * CH) * PC + r] = v;
     });
-  } else {
-    TC* out = (TC*)d.Cout;
-    const TC* cin = (const TC*)d.Cin;
-    const int64_t ldo = d.ldcout, ldi = d.ldcin;
-    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-      int gr = r0 + r, gc = c0 + c;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < (BM * CH) / kThreads; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int r = e % BM, c = e / BM;
+      const int gr = r0 + r, gc = c0 + q * CH + c;
       if (gr < m && gc < n) {
-        double x = alpha * double(v);
-        if (beta != 0.0) x += beta * double(cin[gr + gc * ldi]);
+        double x = alpha * double(cs[c * PC + r]);
+        if (use_beta) {
+          if constexpr (PREFETCH) x += beta * double(cpre[s]);
+          else x += beta * double(cin[gr + gc * ldi]);
+        }
         out[gr + gc * ldo] = TC(x);
       }
-    });
+    }
+    __syncthreads();
   }
 }
 

@@ -54,7 +54,10 @@ struct Launch {
   DT t[4];
   bool trans = false;
   int role = 0;
-  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128
+  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128 (default choice)
+  int gid = -1;          // GEMM launches with both tilings: index into the plan's tile choices
+  size_t off_t[2] = {0, 0};
+  int64_t tiles_t[2] = {0, 0};
   size_t off = 0;        // byte offset of the first descriptor in the blob
   int count = 0;
   int64_t tiles = 0;
@@ -128,6 +131,13 @@ struct hodlr_plan_s {
   struct Pending { int cat; hipEvent_t a, b; double flops, bytes; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
+  // GEMM tilings: per tunable launch the chosen tile (0: 64 x 64, 1: 128 x 128)
+  std::vector<int> tile_choice;
+  bool tuning = false;
+  int tune_cfg = 0;
+  std::vector<double> tune_ms[2];
+  struct TunePending { int gid; hipEvent_t a, b; };
+  std::vector<TunePending> tune_pending;
   // info
   hodlr_plan_info_t info{};
   bool factored = false;
@@ -150,6 +160,8 @@ struct Builder {
   hodlr_plan_s* P;
   Blob blob;
   size_t scratch_need = 0;   // doubles
+  size_t ngid = 0;           // GEMM launches with both tilings
+  bool tunable = false;      // the plan's own launches (one-off batches are not tuned)
 
   char* wsp(size_t off) { return P->ws.p + off; }
 
@@ -166,7 +178,8 @@ struct Builder {
     std::vector<GemmDesc> d;
     std::vector<SumDesc> sums;
     // tile size: 128 x 128 for the big roles when the problems are large enough
-    // and the batch still fills the GPU
+    // and the batch still fills the GPU; launches of roles 1-3 carry both
+    // tilings and the plan's autotuner may switch them
     int tile = 0;
     {
       double sm = 0, sn = 0;
@@ -176,13 +189,11 @@ struct Builder {
         if (g.m > 0 && g.n > 0) { sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128); }
       if (role >= 1 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0)) tile = 1;
     }
-    const int TM = tile ? 128 : kGemmBM, TN = tile ? 128 : kGemmBN;
-    double work = 0;   // tile-rows of K work of the splittable problems
+    double work = 0;   // 64-tile rows of K work (independent of the tiling)
     for (const G& g : gs)
-      if (g.m > 0 && g.n > 0) work += double(cdiv(g.m, TM) * cdiv(g.n, TN)) * std::max(g.k, 1);
+      if (g.m > 0 && g.n > 0) work += double(cdiv(g.m, kGemmBM) * cdiv(g.n, kGemmBN)) * std::max(g.k, 1);
     const int64_t kMinChunk = 1024;
     const int64_t chunk = min_wg > 0 ? std::max<int64_t>(kMinChunk, int64_t(work / double(min_wg))) : 0;
-    int64_t tiles = 0;
     size_t part = 0;  // doubles of split-K partials
     double flops = 0, bytes = 0;
     for (const G& g : gs) {
@@ -191,8 +202,6 @@ struct Builder {
       x.A = g.A; x.B = g.B; x.Cin = g.Cin; x.Cout = g.Cout;
       x.lda = g.lda; x.ldb = g.ldb; x.ldcin = g.ldcin; x.ldcout = g.ldcout;
       x.m = g.m; x.n = g.n; x.k = g.k;
-      x.tiles_m = int(cdiv(g.m, TM));
-      x.tiles_n = int(cdiv(g.n, TN));
       x.alpha = g.alpha; x.beta = g.beta;
       int nsplit = 1;
       if (chunk > 0 && g.beta == 0.0 && tc == F64 && g.k >= 2 * chunk) nsplit = int(cdiv(g.k, chunk));
@@ -214,8 +223,6 @@ struct Builder {
         sums.push_back(sd);
         part += size_t(nsplit) * g.m * g.n;
       }
-      x.tile_begin = tiles;
-      tiles += int64_t(x.tiles_m) * x.tiles_n * nsplit;
       d.push_back(x);
       flops += 2.0 * g.m * g.n * g.k;
       bytes += double(g.m) * g.k * esize(ta) + double(g.k) * g.n * esize(tb) +
@@ -232,8 +239,23 @@ struct Builder {
     L.kind = K_GEMM; L.cat = cat;
     L.t[0] = ta; L.t[1] = tb; L.t[2] = tc; L.t[3] = mt;
     L.trans = trans; L.role = role; L.tile = tile;
-    L.off = blob.add(d); L.count = int(d.size()); L.tiles = tiles;
+    L.count = int(d.size());
     L.flops = flops; L.bytes = bytes;
+    for (int cfg = 0; cfg < (role >= 1 ? 2 : 1); ++cfg) {
+      const int TM = cfg ? 128 : kGemmBM, TN = cfg ? 128 : kGemmBN;
+      int64_t tiles = 0;
+      for (auto& x : d) {
+        x.tiles_m = int(cdiv(x.m, TM));
+        x.tiles_n = int(cdiv(x.n, TN));
+        x.tile_begin = tiles;
+        tiles += int64_t(x.tiles_m) * x.tiles_n * x.nsplit;
+      }
+      L.off_t[cfg] = blob.add(d);
+      L.tiles_t[cfg] = tiles;
+    }
+    if (role >= 1 && tunable) L.gid = int(ngid++);
+    L.off = L.off_t[tile];
+    L.tiles = L.tiles_t[tile];
     pr.l.push_back(L);
     if (!sums.empty()) {
       int64_t st = 0;
@@ -676,8 +698,15 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
   const void* d = P->descs.p + L.off;
   hipStream_t s = P->stream;
   switch (L.kind) {
-    case K_GEMM:
-      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, L.tile, (const GemmDesc*)d, L.count, L.tiles, s);
+    case K_GEMM: {
+      int tile = L.tile;
+      if (L.gid >= 0) {
+        if (P->tuning) tile = P->tune_cfg;
+        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) tile = P->tile_choice[L.gid];
+      }
+      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile,
+                         (const GemmDesc*)(P->descs.p + L.off_t[tile]), L.count, L.tiles_t[tile], s);
+    }
     case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.mmax, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s);
@@ -722,6 +751,15 @@ int harvest(hodlr_plan_s* P) {
 
 int run(hodlr_plan_s* P, const Proc& pr) {
   for (const Launch& L : pr.l) {
+    if (P->tuning && L.kind == K_GEMM && L.gid >= 0) {
+      hipEvent_t a = take_event(P), b = take_event(P);
+      if (!a || !b) return HODLR_ERR_HIP;
+      if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+      if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
+      if (hipEventRecord(b, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+      P->tune_pending.push_back({L.gid, a, b});
+      continue;
+    }
     if (P->prof_mask & (1u << L.cat)) {
       hipEvent_t a = take_event(P), b = take_event(P);
       if (!a || !b) return HODLR_ERR_HIP;
@@ -934,8 +972,10 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
     P->scratch.bytes = P->scratch_doubles * 8;
   }
   Builder B{P.get()};
+  B.tunable = true;
   P->leaf = Proc(); P->extract = Proc(); P->residual = Proc(); P->update = Proc(); P->solve_only_leaf = Proc();
   build_all(P.get(), B);
+  P->tile_choice.assign(B.ngid, -1);
   // the fused pass
   P->factor = Proc();
   P->factor.append(P->leaf);
@@ -1331,6 +1371,45 @@ int hodlr_use_graph(hodlr_plan_t P, int32_t enable) {
   if (!P) return HODLR_ERR_ARG;
   if (enable && P->stream == nullptr) return HODLR_ERR_ARG;
   P->use_graph = enable != 0;
+  return HODLR_OK;
+}
+
+int hodlr_autotune(hodlr_plan_t P, int32_t reps, int32_t* switched) {
+  if (!P) return HODLR_ERR_ARG;
+  const size_t ng = P->tile_choice.size();
+  if (switched) *switched = 0;
+  if (!ng) return HODLR_OK;
+  for (int c = 0; c < 2; ++c) P->tune_ms[c].assign(ng, 0.0);
+  int rc = HODLR_OK;
+  const int R = std::max(reps, 1) + 1;   // the first round only warms up
+  for (int r = 0; r < R && !rc; ++r) {
+    for (int cfg = 0; cfg < 2 && !rc; ++cfg) {
+      P->tuning = true;
+      P->tune_cfg = cfg;
+      rc = run(P, P->factor);
+      if (!rc) rc = solve_only_internal(P);
+      P->tuning = false;
+      if (hipStreamSynchronize(P->stream) != hipSuccess && !rc) rc = HODLR_ERR_HIP;
+      for (auto& q : P->tune_pending) {
+        float ms = 0;
+        if (!rc && hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) rc = HODLR_ERR_HIP;
+        if (r > 0) P->tune_ms[cfg][q.gid] += ms;
+        P->pool.push_back(q.a);
+        P->pool.push_back(q.b);
+      }
+      P->tune_pending.clear();
+    }
+  }
+  if (rc) return rc;
+  int sw = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    int best = P->tune_ms[1][g] < P->tune_ms[0][g] ? 1 : 0;
+    P->tile_choice[g] = best;
+    sw += best;
+  }
+  if (switched) *switched = sw;
+  if (P->graph_exec) { (void)hipGraphExecDestroy(P->graph_exec); P->graph_exec = nullptr; }
+  P->factored = false;   // the tuning runs left the workspace in an arbitrary state
   return HODLR_OK;
 }
 

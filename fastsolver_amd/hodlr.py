@@ -228,6 +228,13 @@ class Plan:
         check(self.lib.hodlr_memcpy(out.ctypes.data, p, out.nbytes, 1), "workspace download")
         return out
 
+    def autotune(self, reps=2):
+        """Pick 64x64 or 128x128 tiles per GEMM launch by timing both; returns
+        how many launches chose 128x128.  Factor again afterwards."""
+        big = ctypes.c_int32()
+        check(self.lib.hodlr_autotune(self.h, reps, ctypes.byref(big)), "hodlr_autotune")
+        return big.value
+
     def use_graph(self, enable=True):
         check(self.lib.hodlr_use_graph(self.h, int(enable)), "hodlr_use_graph")
 

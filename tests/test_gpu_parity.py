@@ -191,3 +191,17 @@ def test_singular_leaf_reports_status():
 
 def hodlr_err_singular():
     return -6
+
+
+def test_autotune_then_solve_matches():
+    """The autotuner may switch tilings; a fresh factor+solve gives the same answer
+    to rounding."""
+    spec = gen.config(2, scale=16)[0]
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_FP64)
+    plan.autotune(1)
+    plan.factor_solve()
+    plan.synchronize()
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
