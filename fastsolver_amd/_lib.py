@@ -53,6 +53,7 @@ DEVICE_SYMBOLS = {
     "hodlr_workspace": ([_ptr, _i32, ctypes.POINTER(_ptr), ctypes.POINTER(_i64), ctypes.POINTER(_i32)], ctypes.c_int),
     "hodlr_use_graph": ([_ptr, _i32], ctypes.c_int),
     "hodlr_autotune": ([_ptr, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
+    "hodlr_tune_force": ([_ptr, _i32], ctypes.c_int),
     "hodlr_profile_enable": ([_ptr, _i32], ctypes.c_int),
     "hodlr_profile_reset": ([_ptr], ctypes.c_int),
     "hodlr_profile_read": ([_ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),

@@ -276,23 +276,31 @@ int hodlr_gen_random(int64_t n, int64_t threshold, int nrhs, int nranks, const i
   // decaying spectrum: U = Q_U diag(10^(-k/decay)), V = Q_V.  Small blocks are
   // orthonormalised in parallel (one BLAS thread each), big ones one at a time.
   const int nthr = omp_get_max_threads();
-  auto one = [&](int id, int which) -> int {
+  auto one = [&](int id, int which, bool parallel) -> int {
     const hodlr::TreeNode& x = t.nodes[id];
     const int r = t.rank(x.depth);
     const int nn = int(x.nrow);
     Stream s(seed, sys, which == 0 ? KIND_U : KIND_V, uint32_t(id));
     std::vector<double> Q(size_t(nn) * r);
-    for (int64_t e = 0; e < int64_t(Q.size()); ++e) Q[e] = s.normal(uint64_t(e));
+    const int64_t ne = int64_t(Q.size());
+    if (parallel) {
+#pragma omp parallel for schedule(static)
+      for (int64_t e = 0; e < ne; ++e) Q[e] = s.normal(uint64_t(e));
+    } else {
+      for (int64_t e = 0; e < ne; ++e) Q[e] = s.normal(uint64_t(e));
+    }
     int info = thin_q(nn, r, Q.data());
     if (info) return info;
     if (which == 0) {
       const int64_t ucol = t.col_beg(x.depth);
+#pragma omp parallel for schedule(static) if (parallel)
       for (int q = 0; q < r; ++q) {
         double sig = pow(10.0, -double(q) / fam->decay);
         for (int a = 0; a < nn; ++a) X[x.row_beg + a + (ucol + q) * ldx] = Q[a + size_t(q) * nn] * sig;
       }
     } else {
       const int64_t vcol = t.rsum[x.depth - 1];
+#pragma omp parallel for schedule(static) if (parallel)
       for (int q = 0; q < r; ++q)
         for (int a = 0; a < nn; ++a) V[x.row_beg + a + (vcol + q) * ldv] = Q[a + size_t(q) * nn];
     }
@@ -301,13 +309,13 @@ int hodlr_gen_random(int64_t n, int64_t threshold, int nrhs, int nranks, const i
   std::vector<int> small, big;
   for (int id : order) (t.nodes[id].nrow * t.rank(t.nodes[id].depth) < (int64_t(1) << 22) ? small : big).push_back(id);
   for (int id : big)
-    for (int w = 0; w < 2; ++w) { int info = one(id, w); if (info && !status) status = info; }
+    for (int w = 0; w < 2; ++w) { int info = one(id, w, true); if (info && !status) status = info; }
 #pragma omp parallel num_threads(nthr)
   {
     if (g_qr.local_threads) g_qr.local_threads(1);
 #pragma omp for schedule(dynamic, 1)
     for (int64_t k = 0; k < int64_t(small.size()) * 2; ++k) {
-      int info = one(small[k / 2], int(k % 2));
+      int info = one(small[k / 2], int(k % 2), false);
       if (info) {
 #pragma omp critical
         if (!status) status = info;

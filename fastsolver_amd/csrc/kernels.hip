@@ -503,6 +503,120 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restri
   }
 }
 
+// TRSM with 128-row steps: the product with the rows already solved runs as
+// one 128 x CB MFMA tile (half the re-reads of solved rows of 64-row steps),
+// then the two 64-row halves are finished with the inverted 64 x 64 diagonal
+// blocks and the 64 x 64 coupling block between them.  CB = 64 (fp64) or 128
+// (fp32) columns per workgroup.
+template <class TLU, class TIN, class TOUT, int CB>
+__global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+  typedef TLU MT;
+  constexpr int BK = kGemmBK, RB = 128, HB = kDB, PT = CB + 1, PI = HB + 1;
+  typedef Tile<MT, RB, CB, BK> Big;
+  typedef Tile<MT, HB, CB, BK> Half;
+  constexpr int SU = (Big::LDS_ELEMS > HB * PI) ? Big::LDS_ELEMS : HB * PI;
+  __shared__ MT s_u[SU];       // MFMA stages, then a 64 x 64 block in As layout s_u[k*PI + m]
+  __shared__ MT s_t[RB * PT];  // right-hand-side rows of the step, s_t[r*PT + c]
+
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+  const TrsmDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int c0 = int(bid - d.tile_begin) * CB;
+  const int m = d.m, ncols = d.ncols;
+  const int nc = min(CB, ncols - c0);
+  const TLU* LU = (const TLU*)d.LU;
+  const TLU* DI = (const TLU*)d.dinv;
+  const TIN* Xin = (const TIN*)d.Xin;
+  TOUT* Xout = (TOUT*)d.Xout;
+  const int64_t ldlu = d.ldlu, ldin = d.ldin, ldout = d.ldout;
+  const int nstep = (m + RB - 1) / RB;
+
+  // s_u <- 64 x 64 block of LU at (r0, k0) (zero outside nr x nk)
+  auto load_block = [&](int r0, int k0, int nr, int nk) {
+    for (int e = threadIdx.x; e < HB * HB; e += kThreads) {
+      const int i = e % HB, k = e / HB;
+      s_u[k * PI + i] = (i < nr && k < nk) ? MT(LU[(r0 + i) + int64_t(k0 + k) * ldlu]) : MT(0);
+    }
+  };
+  auto load_inv = [&](int b, int which) {
+    const TLU* inv = DI + int64_t(b) * 2 * HB * HB + which * HB * HB;
+    for (int e = threadIdx.x; e < HB * HB; e += kThreads) s_u[(e / HB) * PI + e % HB] = inv[e];
+  };
+  // rows [h*64, h*64+64) of s_t <- inv * rows of s_t; also written to Xout
+  auto finish_half = [&](int h, int R0, int nr_h) {
+    Half t;
+    t.zero();
+    t.template mma<HB>(s_u, s_t + h * HB * PT, PI, PT);
+    __syncthreads();   // every wave has read the half before it is overwritten
+    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+      s_t[(h * HB + r) * PT + c] = v;
+      if (r < nr_h && c < nc) Xout[(R0 + h * HB + r) + int64_t(c0 + c) * ldout] = TOUT(v);
+    });
+    __syncthreads();
+  };
+  // rows of half `dst` of s_t -= block(s_u) * rows of half `src`
+  auto couple = [&](int dst, int src) {
+    Half t;
+    t.zero();
+    t.template mma<HB>(s_u, s_t + src * HB * PT, PI, PT);
+    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[(dst * HB + r) * PT + c] -= v; });
+    __syncthreads();
+  };
+
+  for (int pass = 0; pass < 2; ++pass) {      // 0: forward with L, 1: backward with U
+    for (int s = 0; s < nstep; ++s) {
+      const int st = pass == 0 ? s : nstep - 1 - s;
+      const int R0 = st * RB, nr = min(RB, m - R0);
+      const int n0 = min(HB, nr), n1 = nr - n0;   // rows of the two halves
+      {
+        Big t;
+        t.zero();
+        if (pass == 0)
+          gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, R0, m, Xout, ldout, c0, ncols, 0, R0);
+        else
+          gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, R0, R0 + nr, Xout, ldout, c0, ncols, R0 + RB, m);
+        t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+          MT x = 0;
+          if (r < nr && c < nc) {
+            if (pass == 0) {
+              const int src = d.perm ? d.perm[R0 + r] : R0 + r;
+              x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
+            } else {
+              x = MT(Xout[(R0 + r) + int64_t(c0 + c) * ldout]) - v;
+            }
+          }
+          s_t[r * PT + c] = x;
+        });
+      }
+      const int b0 = 2 * st;   // 64-row diagonal block of the first half
+      if (pass == 0) {
+        load_inv(b0, 0);
+        __syncthreads();
+        finish_half(0, R0, n0);
+        if (n1 > 0) {
+          load_block(R0 + HB, R0, n1, n0);   // L coupling block
+          __syncthreads();
+          couple(1, 0);
+          load_inv(b0 + 1, 0);
+          __syncthreads();
+          finish_half(1, R0, n1);
+        }
+      } else {
+        if (n1 > 0) {
+          load_inv(b0 + 1, 1);
+          __syncthreads();
+          finish_half(1, R0, n1);
+          load_block(R0, R0 + HB, n0, n1);   // U coupling block
+          __syncthreads();
+          couple(0, 1);
+        }
+        load_inv(b0, 1);
+        __syncthreads();
+        finish_half(0, R0, n0);
+      }
+    }
+  }
+}
+
 // ====================================================================== norms
 __global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restrict__ descs, int ndesc) {
   const int64_t bid = blockIdx.x;
@@ -626,13 +740,24 @@ hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_trsm(DT tlu, DT tin, DT tout, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_trsm(DT tlu, DT tin, DT tout, int variant, const TrsmDesc* d, int n, int64_t tiles,
+                       hipStream_t s) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
-  if (tlu == F64 && tin == F64 && tout == F64) hipLaunchKernelGGL((trsm_kernel<double, double, double>), g, b, 0, s, d, n);
-  else if (tlu == F32 && tin == F64 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, double, float>), g, b, 0, s, d, n);
-  else if (tlu == F32 && tin == F32 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float>), g, b, 0, s, d, n);
-  else return hipErrorInvalidValue;
+  if (variant == 0) {
+    if (tlu == F64 && tin == F64 && tout == F64) hipLaunchKernelGGL((trsm_kernel<double, double, double>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F64 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, double, float>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F32 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float>), g, b, 0, s, d, n);
+    else return hipErrorInvalidValue;
+  } else {
+    if (tlu == F64 && tin == F64 && tout == F64)
+      hipLaunchKernelGGL((trsm128_kernel<double, double, double, 64>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F64 && tout == F32)
+      hipLaunchKernelGGL((trsm128_kernel<float, double, float, 128>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F32 && tout == F32)
+      hipLaunchKernelGGL((trsm128_kernel<float, float, float, 128>), g, b, 0, s, d, n);
+    else return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

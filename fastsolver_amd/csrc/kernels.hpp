@@ -28,7 +28,10 @@ hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
 hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s);  // inverted 64x64 diagonal blocks
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
 // (TLU, TIN, TOUT) in {(F64,F64,F64), (F32,F64,F32), (F32,F32,F32)}; compute type = TLU
-hipError_t launch_trsm(DT tlu, DT tin, DT tout, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s);
+// variant 0: 64-row steps, 64-column panels; 1: 128-row steps, trsm_panel_cols(tlu)-column panels
+hipError_t launch_trsm(DT tlu, DT tin, DT tout, int variant, const TrsmDesc* d, int n, int64_t tiles,
+                       hipStream_t s);
+inline int trsm_panel_cols(DT tlu, int variant) { return variant == 0 ? kTrsmCB : (tlu == F64 ? 64 : 128); }
 hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s);
 

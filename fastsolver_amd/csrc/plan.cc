@@ -73,6 +73,7 @@ struct Proc {
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline size_t esize(DT t) { return t == F64 ? 8 : 4; }
 
+
 // Device buffer with typed offsets.
 struct Buf {
   char* p = nullptr;
@@ -187,7 +188,8 @@ struct Builder {
       int64_t big_tiles = 0;
       for (const G& g : gs)
         if (g.m > 0 && g.n > 0) { sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128); }
-      if (role >= 1 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0)) tile = 1;
+      if (role >= 1 && mt == F32 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0))
+        tile = 1;
     }
     double work = 0;   // 64-tile rows of K work (independent of the tiling)
     for (const G& g : gs)
@@ -337,14 +339,13 @@ struct Builder {
     pr.l.push_back(Dv);
   }
 
+  // Both TRSM variants (64-row steps / 128-row steps) get descriptor sets;
+  // the plan's launches are tunable like the GEMM tilings.
   void trsm(Proc& pr, Cat cat, DT tlu, DT tin, DT tout, std::vector<TrsmDesc> v) {
-    int64_t t = 0;
     std::vector<TrsmDesc> d;
     double flops = 0, bytes = 0;
     for (auto& x : v) {
       if (x.m <= 0 || x.ncols <= 0) continue;
-      x.tile_begin = t;
-      t += cdiv(x.ncols, kTrsmCB);
       flops += 2.0 * double(x.m) * x.m * x.ncols;
       bytes += double(x.m) * x.m * esize(tlu) + double(x.m) * x.ncols * (esize(tin) + esize(tout));
       d.push_back(x);
@@ -352,7 +353,20 @@ struct Builder {
     if (d.empty()) return;
     Launch L;
     L.kind = K_TRSM; L.cat = cat; L.t[0] = tlu; L.t[1] = tin; L.t[2] = tout;
-    L.off = blob.add(d); L.count = int(d.size()); L.tiles = t; L.flops = flops; L.bytes = bytes;
+    L.count = int(d.size()); L.flops = flops; L.bytes = bytes;
+    for (int var = 0; var < 2; ++var) {
+      int64_t t = 0;
+      for (auto& x : d) {
+        x.tile_begin = t;
+        t += cdiv(x.ncols, trsm_panel_cols(tlu, var));
+      }
+      L.off_t[var] = blob.add(d);
+      L.tiles_t[var] = t;
+    }
+    L.tile = 1;
+    if (tunable) L.gid = int(ngid++);
+    L.off = L.off_t[L.tile];
+    L.tiles = L.tiles_t[L.tile];
     pr.l.push_back(L);
   }
 
@@ -714,7 +728,15 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_ROW: return launch_lu_rowblock(L.t[0], (const LuDesc*)d, L.count, L.k0, L.mmax, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
-    case K_TRSM: return launch_trsm(L.t[0], L.t[1], L.t[2], (const TrsmDesc*)d, L.count, L.tiles, s);
+    case K_TRSM: {
+      int var = L.tile;
+      if (L.gid >= 0) {
+        if (P->tuning) var = P->tune_cfg;
+        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) var = P->tile_choice[L.gid];
+      }
+      return launch_trsm(L.t[0], L.t[1], L.t[2], var, (const TrsmDesc*)(P->descs.p + L.off_t[var]), L.count,
+                         L.tiles_t[var], s);
+    }
     case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s);
     case K_NORM_FIN: return launch_norm_finish((const NormDesc*)d, L.count, s);
   }
@@ -751,7 +773,7 @@ int harvest(hodlr_plan_s* P) {
 
 int run(hodlr_plan_s* P, const Proc& pr) {
   for (const Launch& L : pr.l) {
-    if (P->tuning && L.kind == K_GEMM && L.gid >= 0) {
+    if (P->tuning && (L.kind == K_GEMM || L.kind == K_TRSM) && L.gid >= 0) {
       hipEvent_t a = take_event(P), b = take_event(P);
       if (!a || !b) return HODLR_ERR_HIP;
       if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
@@ -1410,6 +1432,13 @@ int hodlr_autotune(hodlr_plan_t P, int32_t reps, int32_t* switched) {
   if (switched) *switched = sw;
   if (P->graph_exec) { (void)hipGraphExecDestroy(P->graph_exec); P->graph_exec = nullptr; }
   P->factored = false;   // the tuning runs left the workspace in an arbitrary state
+  return HODLR_OK;
+}
+
+int hodlr_tune_force(hodlr_plan_t P, int32_t cfg) {
+  if (!P || cfg < -1 || cfg > 1) return HODLR_ERR_ARG;
+  for (auto& c : P->tile_choice) c = cfg;
+  if (P->graph_exec) { (void)hipGraphExecDestroy(P->graph_exec); P->graph_exec = nullptr; }
   return HODLR_OK;
 }
 

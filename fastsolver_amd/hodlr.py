@@ -235,6 +235,11 @@ class Plan:
         check(self.lib.hodlr_autotune(self.h, reps, ctypes.byref(big)), "hodlr_autotune")
         return big.value
 
+    def force_tiling(self, cfg):
+        """0: 64x64 GEMM tiles and 64-row TRSM steps everywhere, 1: the larger
+        choices, -1: built-in defaults."""
+        check(self.lib.hodlr_tune_force(self.h, cfg), "hodlr_tune_force")
+
     def use_graph(self, enable=True):
         check(self.lib.hodlr_use_graph(self.h, int(enable)), "hodlr_use_graph")
 

@@ -162,12 +162,16 @@ int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, do
  * reference's U region after the solve), element type 0 = double, 1 = float. */
 int hodlr_workspace(hodlr_plan_t plan, int32_t sys, void** ptr, int64_t* ld, int32_t* dtype);
 
-/* Tile autotuner: times every large GEMM launch of the fused pass and of the
- * correction solve with 64 x 64 and with 128 x 128 workgroup tiles (`reps`
- * timed rounds after one warm-up round) and keeps the faster tiling per
- * launch.  Leaves the workspace in an arbitrary state: factor again after.
- * `big` (may be NULL) returns how many launches chose 128 x 128. */
+/* Autotuner: times every large GEMM launch of the fused pass and of the
+ * correction solve with 64 x 64 and with 128 x 128 workgroup tiles, and every
+ * TRSM launch with 64-row and with 128-row steps (`reps` timed rounds after
+ * one warm-up round), and keeps the faster choice per launch.  Leaves the
+ * workspace in an arbitrary state: factor again after.  `big` (may be NULL)
+ * returns how many launches chose the larger tiling. */
 int hodlr_autotune(hodlr_plan_t plan, int32_t reps, int32_t* big);
+/* Force every tunable launch to the smaller (0) or larger (1) choice; -1
+ * restores the built-in defaults. */
+int hodlr_tune_force(hodlr_plan_t plan, int32_t cfg);
 
 /* Record the fused pass in a hipGraph and replay it on later calls
  * (needs a non-default stream). */

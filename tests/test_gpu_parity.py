@@ -205,3 +205,18 @@ def test_autotune_then_solve_matches():
     plan.factor_solve()
     plan.synchronize()
     assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("prec", [hodlr.PREC_FP64, hodlr.PREC_FP32])
+def test_every_kernel_variant(cfg, prec):
+    """Both GEMM tilings and both TRSM variants, odd sizes included."""
+    spec = gen.SystemSpec(6000, 400, 5, [70, 40, 33, 20], d_scale=1 / 16)
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], prec)
+    plan.force_tiling(cfg)
+    rep = plan.solve(tol=TOL, maxiter=10)
+    assert rep.converged
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= (TOL if prec == hodlr.PREC_FP64 else 1e-11)
