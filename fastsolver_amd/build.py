@@ -5,6 +5,7 @@
 Produces, next to this file:
   libfastsolver_amd.so      HIP kernels + plan + C-ABI (include/fastsolver_amd.h)
   libfastsolver_amd_gen.so  host input generation (include/fastsolver_amd_gen.h)
+  examples/single_launch    C++ driver over include/fastsolver_amd.hpp
 """
 import os
 import shutil
@@ -25,6 +26,8 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wno-unuse
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 DEVICE_LIB = os.path.join(HERE, "libfastsolver_amd.so")
 GEN_LIB = os.path.join(HERE, "libfastsolver_amd_gen.so")
+EXAMPLES = os.path.join(HERE, "examples")
+EXAMPLE_BINS = ["single_launch"]
 
 
 def _run(cmd):
@@ -60,9 +63,24 @@ def build_gen():
     return GEN_LIB
 
 
+def build_examples():
+    """C++ drivers linked against the two libraries (rpath $ORIGIN/..)."""
+    out = []
+    for name in EXAMPLE_BINS:
+        exe = os.path.join(EXAMPLES, name)
+        tmp = exe + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+              os.path.join(EXAMPLES, name + ".cc"), "-L", HERE, "-lfastsolver_amd", "-lfastsolver_amd_gen",
+              "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+        os.replace(tmp, exe)
+        out.append(exe)
+    return out
+
+
 def build_all():
     build_gen()
     build_device()
+    build_examples()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,255 @@
+// C++ facade of fastsolver_amd: the reference's driver API (HodlrMatrix,
+// FastSolver, Range) without the Legion Context / HighLevelRuntime arguments,
+// over the C-ABI of libfastsolver_amd.so and libfastsolver_amd_gen.so.
+//
+//   reference                                         here
+//   HodlrMatrix(col,row,gl,sl,r,t,leaf,name)          same  (include/htree/hodlr_matrix.h:19-21)
+//   create_tree(ctx, rt)                              create_tree()          (hodlr_matrix.h:24-25)
+//   init_rhs(seed, range, ctx, rt)                    init_rhs(seed, range)  (hodlr_matrix.h:26-27)
+//   init_circulant_matrix(diag, range, ctx, rt, skipU) init_circulant_matrix(diag, range, skipU)
+//   save_rhs / save_solution                          same (text format of SaveRegionTask)
+//   launch_level, get_num_rhs, get_num_leaf, get_file_soln   same
+//   FastSolver::bfs_solve(hmat, range, ctx, rt)       bfs_solve(hmat, range) (fast_solver.h:19-20)
+//   register_solver_tasks()                           register_solver_tasks(): checks a GPU is present
+//
+// The matrix lives in device memory owned by the HodlrMatrix; the solve is
+// not destructive (the reference overwrites K and U, hodlr_matrix.cc / solver_tasks.cc:437).
+#ifndef FASTSOLVER_AMD_HPP
+#define FASTSOLVER_AMD_HPP
+
+#include <stdint.h>
+
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fastsolver_amd.h"
+#include "fastsolver_amd_gen.h"
+
+namespace fastsolver_amd {
+
+inline void check(int rc, const char* what) {
+  if (rc != HODLR_OK) throw std::runtime_error(std::string(what) + ": " + hodlr_error_string(rc));
+}
+
+// Placement hint of the reference (include/core/range.h); one GPU ignores it.
+class Range {
+ public:
+  Range() : mbegin(0), msize(0) {}
+  explicit Range(int size) : mbegin(0), msize(size) {}
+  Range(int begin, int size) : mbegin(begin), msize(size) {}
+  int begin() const { return mbegin; }
+  int size() const { return msize; }
+  Range lchild() const { return Range(mbegin, msize / 2); }
+  Range rchild() const { return Range(mbegin + msize / 2, msize / 2); }
+  int mbegin, msize;
+};
+
+// Device buffer owned by the facade.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(int64_t bytes) { check(hodlr_device_malloc(&p_, bytes), "hodlr_device_malloc"); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    release();
+    p_ = o.p_;
+    o.p_ = nullptr;
+    return *this;
+  }
+  ~DeviceBuffer() { release(); }
+  void* get() const { return p_; }
+  void upload(const void* h, int64_t bytes) { check(hodlr_memcpy(p_, h, bytes, 0), "upload"); }
+  void download(void* h, int64_t bytes) const { check(hodlr_memcpy(h, p_, bytes, 1), "download"); }
+
+ private:
+  void release() {
+    if (p_) hodlr_device_free(p_);
+    p_ = nullptr;
+  }
+  void* p_ = nullptr;
+};
+
+class HodlrMatrix {
+ public:
+  HodlrMatrix() = default;
+  HodlrMatrix(int col, int row, int gl, int sl, int r, int t, int leaf, const std::string& name)
+      : rhs_cols(col), rhs_rows(row), gloLevel(gl), subLevel(sl), rank(r), threshold(t), leafSize(leaf),
+        file_rhs(name + "_rhs.txt"), file_soln(name + "_soln.txt") {}
+
+  // Builds the tree (hodlr_matrix.cc:44-82) and allocates the device arrays.
+  void create_tree() {
+    int nleaves = 0, maxdepth = 0;
+    int64_t xcols = 0, vcols = 0, kpacked = 0;
+    int32_t r = rank;
+    check(hodlr_gen_tree(rhs_rows, threshold, rhs_cols, 1, &r, &nleaves, &maxdepth, &xcols, &vcols, &kpacked),
+          "create_tree");
+    nLegionLeaf = nleaves;
+    depth = maxdepth;
+    ncolX = xcols;
+    ncolV = vcols;
+    nK = kpacked;
+    X = DeviceBuffer(int64_t(rhs_rows) * xcols * 8);
+    V = DeviceBuffer(int64_t(rhs_rows) * std::max<int64_t>(vcols, 1) * 8);
+    K = DeviceBuffer(kpacked * 8);
+    x = DeviceBuffer(int64_t(rhs_rows) * rhs_cols * 8);
+  }
+
+  // The reseeded drand48 right-hand side (hodlr_matrix.cc:175-209).  The data is
+  // generated together with the matrix by init_circulant_matrix.
+  void init_rhs(long seed, const Range& = Range()) { rhs_seed = seed; }
+
+  // A = diag*I + U U^T with U(i,j) = (i+j) mod rank (hodlr_matrix.cc:94-109).
+  void init_circulant_matrix(double diag, const Range& = Range(), bool /*skipU*/ = false) {
+    std::vector<double> hX(size_t(rhs_rows) * ncolX);
+    std::vector<double> hV(size_t(rhs_rows) * std::max<int64_t>(ncolV, 1));
+    std::vector<double> hK(static_cast<size_t>(nK));
+    check(hodlr_gen_circulant(rhs_rows, threshold, rhs_cols, rank, diag, rhs_seed, 0, hX.data(), rhs_rows,
+                              hV.data(), rhs_rows, hK.data(), 0, HODLR_K_PACKED),
+          "init_circulant_matrix");
+    X.upload(hX.data(), int64_t(hX.size()) * 8);
+    V.upload(hV.data(), int64_t(hV.size()) * 8);
+    K.upload(hK.data(), int64_t(hK.size()) * 8);
+  }
+
+  // Text dumps in SaveRegionTask's format (save_region_task.cc:37-67): per leaf
+  // region, optionally the seed, then one row per line of tab-terminated values.
+  void save_rhs() const { save(file_rhs, true); }
+  void save_solution() const { save(file_soln, false); }
+
+  int launch_level() const { return gloLevel - subLevel; }
+  int get_num_rhs() const { return rhs_cols; }
+  int get_num_leaf() const { return nLegionLeaf; }
+  std::string get_file_soln() const { return file_soln; }
+  int rows() const { return rhs_rows; }
+  int levels() const { return depth; }
+
+  hodlr_system_t system() const {
+    hodlr_system_t s{};
+    s.n = rhs_rows;
+    s.threshold = threshold;
+    s.nrhs = rhs_cols;
+    s.nranks = 1;
+    s.ranks = &rank;
+    s.X = (const double*)X.get();
+    s.ldx = rhs_rows;
+    s.V = (const double*)V.get();
+    s.ldv = rhs_rows;
+    s.K = (const double*)K.get();
+    s.klayout = HODLR_K_PACKED;
+    s.x = (double*)x.get();
+    s.ldxs = rhs_rows;
+    return s;
+  }
+
+  // solution x (rows x rhs columns, column-major)
+  std::vector<double> solution() const {
+    std::vector<double> h(size_t(rhs_rows) * rhs_cols);
+    x.download(h.data(), int64_t(h.size()) * 8);
+    return h;
+  }
+  std::vector<double> rhs() const {
+    std::vector<double> h(size_t(rhs_rows) * rhs_cols);
+    X.download(h.data(), int64_t(h.size()) * 8);
+    return h;
+  }
+
+ private:
+  void save(const std::string& file, bool seed) const {
+    std::vector<double> h = seed ? rhs() : solution();
+    std::vector<int64_t> rb(nLegionLeaf), nr(nLegionLeaf);
+    std::vector<int> dp(nLegionLeaf);
+    int32_t r = rank;
+    check(hodlr_gen_leaves(rhs_rows, threshold, rhs_cols, 1, &r, rb.data(), nr.data(), dp.data()), "save");
+    std::ofstream out(file.c_str());
+    for (int l = 0; l < nLegionLeaf; ++l) {
+      if (seed) out << rhs_seed << std::endl;
+      for (int64_t i = rb[l]; i < rb[l] + nr[l]; ++i) {
+        for (int j = 0; j < rhs_cols; ++j) out << std::setprecision(20) << h[i + size_t(j) * rhs_rows] << '\t';
+        out << std::endl;
+      }
+    }
+  }
+
+  int rhs_cols = 0, rhs_rows = 0, gloLevel = 0, subLevel = 0;
+  int32_t rank = 0;
+  int threshold = 0, leafSize = 1, nLegionLeaf = 0, depth = 0;
+  int64_t ncolX = 0, ncolV = 0, nK = 0;
+  long rhs_seed = 0;
+  std::string file_rhs, file_soln;
+  DeviceBuffer X, V, K, x;
+};
+
+inline void register_solver_tasks() {
+  int32_t n = 0;
+  check(hodlr_device_count(&n), "register_solver_tasks");
+  if (n < 1) throw std::runtime_error("register_solver_tasks: no GPU visible");
+}
+
+class FastSolver {
+ public:
+  explicit FastSolver(int precision = HODLR_PREC_FP64) : precision_(precision) {}
+  FastSolver(const FastSolver&) = delete;
+  FastSolver& operator=(const FastSolver&) = delete;
+  ~FastSolver() { release(); }
+
+  // Factor + solve (fast_solver.cc:56-72); FP32 plans add FP64 refinement to `tol`.
+  void bfs_solve(HodlrMatrix& m, const Range& = Range(), double tol = 1e-12, int maxiter = 20) {
+    release();
+    check(hodlr_stream_create(&stream_), "hodlr_stream_create");
+    hodlr_system_t s = m.system();
+    check(hodlr_plan_create(&plan_, &s, 1, precision_, stream_), "hodlr_plan_create");
+    hodlr_event_create(&t0_);
+    hodlr_event_create(&t1_);
+    hodlr_event_record(t0_, stream_);
+    if (precision_ == HODLR_PREC_FP64) {
+      check(hodlr_factor_solve(plan_), "hodlr_factor_solve");
+    } else {
+      check(hodlr_solve(plan_, tol, maxiter, &report_), "hodlr_solve");
+    }
+    hodlr_event_record(t1_, stream_);
+    check(hodlr_stream_synchronize(stream_), "bfs_solve");
+    hodlr_event_elapsed_ms(t0_, t1_, &time_ms_);
+    int32_t nn = 0;
+    int rc = hodlr_status(plan_, nullptr, nullptr, &nn);
+    check(rc, "bfs_solve status");
+  }
+
+  // max over rhs columns of ||b - A x|| / ||b||
+  double residual() {
+    double mx = 0;
+    check(hodlr_residual(plan_, nullptr, &mx), "hodlr_residual");
+    return mx;
+  }
+
+  void display_launch_time() const {
+    std::cout << "Time for the GPU factor+solve : " << time_ms_ / 1e3 << " s" << std::endl;
+  }
+  double elapsed_ms() const { return time_ms_; }
+  const hodlr_solve_report_t& report() const { return report_; }
+
+ private:
+  void release() {
+    if (plan_) hodlr_plan_destroy(plan_);
+    if (t0_) hodlr_event_destroy(t0_);
+    if (t1_) hodlr_event_destroy(t1_);
+    if (stream_) hodlr_stream_destroy(stream_);
+    plan_ = nullptr;
+    t0_ = t1_ = stream_ = nullptr;
+  }
+  int precision_;
+  hodlr_plan_t plan_ = nullptr;
+  void *stream_ = nullptr, *t0_ = nullptr, *t1_ = nullptr;
+  double time_ms_ = 0;
+  hodlr_solve_report_t report_{};
+};
+
+}  // namespace fastsolver_amd
+
+#endif

@@ -76,3 +76,20 @@ def test_uneven_batch_specs():
 def test_tree_errors():
     with pytest.raises(ValueError):
         gen.tree_info(gen.SystemSpec(200, 50, 1, [60]))   # leaf rows (50) <= rank
+
+
+def test_cxx_facade_compiles_and_example_links():
+    """The C++ facade (the reference's HodlrMatrix / FastSolver driver API)
+    compiles against both headers, and the in-tree example resolves both
+    libraries without LD_LIBRARY_PATH."""
+    import subprocess
+    src = "#include \"fastsolver_amd.hpp\"\nint main(){fastsolver_amd::HodlrMatrix m(2,480,3,3,30,60,1,\"g\");" \
+          "return m.get_num_rhs()==2 ? 0 : 1;}\n"
+    p = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        "-x", "c++", "-"], input=src, text=True, capture_output=True)
+    assert p.returncode == 0, p.stderr
+    exe = os.path.join(ROOT, "fastsolver_amd", "examples", "single_launch")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libfastsolver_amd.so" in ldd and "libfastsolver_amd_gen.so" in ldd
+    assert "not found" not in ldd, ldd
