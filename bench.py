@@ -50,6 +50,22 @@ def timed_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, out
 
 
+def measured_traffic(cat, cfg):
+    """HBM bytes per launch of `cat` on workload cfg<cfg>, from the calibrated
+    rocprofv3 PMC summaries (lines 'TRAFFIC <category> cfg<k> <bytes>')."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for f in sorted(os.listdir(pdir)):
+        if not f.endswith((".md", ".txt")):
+            continue
+        for line in open(os.path.join(pdir, f)):
+            w = line.split()
+            if len(w) == 4 and w[0] == "TRAFFIC" and w[1] == cat and w[2] == f"cfg{cfg}":
+                return float(w[3]), "profiles/" + f
+    return None, None
+
+
 def roofline_entry(prof, cat, precision):
     d = prof[cat]
     if d["launches"] == 0 or d["ms"] <= 0:
@@ -134,6 +150,7 @@ def main():
 
     # ------------------------------------------------------------ mixed + refine
     p32 = hodlr.Plan([ds], hodlr.PREC_FP32)
+    ws32 = p32.info().workspace_bytes
     if args.tune:
         big = p32.autotune(args.tune)
         log(f"[bench] mixed autotune: {big} launches on 128x128 tiles")
@@ -163,7 +180,9 @@ def main():
     log(f"[bench] mixed: {tmix * 1e3:.2f} ms/solve, {rep.iterations} refinement sweeps, "
         f"relres {rep.relres:.2e} (first {rep.relres_first:.2e}); dominant kernel {dom}")
     roof = roofline_entry(prof_dom, dom, "fp32" if dom not in ("node_gemm", "node_lu", "node_trsm", "matvec",
-                                                                "reduce_sum", "norm") else "fp64")
+                                                                "reduce_sum", "norm", "solve_node") else "fp64")
+    if roof is not None:
+        roof["traffic"], roof["traffic_source"] = measured_traffic(dom, args.config)
     value = 1.0 / tmix
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -179,6 +198,8 @@ def main():
                  "gflops": round(info64.flops_factor / t64 / 1e9, 1)},
         "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
         "flops_factor": info64.flops_factor,
+        "launches_fp64_pass": info64.launches_factor,
+        "workspace_gb": {"fp64": round(info64.workspace_bytes / 1e9, 2), "mixed": round(ws32 / 1e9, 2)},
         "roofline": roof,
         "profile_fp64_ms": {k: round(v["ms"], 3) for k, v in prof64.items() if v["launches"]},
         "profile_mixed_ms": {k: round(v["ms"], 3) for k, v in prof_all.items() if v["launches"]},

@@ -39,11 +39,13 @@ constexpr int kMaxBlock = 1024;   // largest leaf / rank the LU panel kernel hol
 enum Cat {
   CAT_LEAF_COPY = 0, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, CAT_LEAF_TRSM,
   CAT_REDUCE, CAT_REDUCE_SUM, CAT_NODE_GEMM, CAT_NODE_LU, CAT_NODE_TRSM, CAT_BCAST,
-  CAT_COPY, CAT_MATVEC, CAT_NORM, CAT_OTHER, CAT_N
+  CAT_COPY, CAT_MATVEC, CAT_NORM, CAT_OTHER, CAT_SOLVE_TRSM, CAT_SOLVE_REDUCE, CAT_SOLVE_NODE, CAT_SOLVE_BCAST,
+  CAT_N
 };
 const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_update", "lu_misc", "leaf_trsm",
                                 "reduce_gemm", "reduce_sum", "node_gemm", "node_lu", "node_trsm", "bcast_gemm",
-                                "copy", "matvec", "norm", "other"};
+                                "copy", "matvec", "norm", "other", "solve_trsm", "solve_reduce", "solve_node",
+                                "solve_bcast"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
 enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_ROW, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN };
@@ -523,7 +525,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, kc);
     B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus);
     B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr);
-    B.trsm(P->solve_only_leaf, CAT_LEAF_TRSM, wt, F64, wt, trs);
+    B.trsm(P->solve_only_leaf, CAT_SOLVE_TRSM, wt, F64, wt, trs);
   }
 
   // ---------------------------------------------------------------- heights
@@ -612,15 +614,19 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       Proc& pr = mode == 0 ? P->reduce[h] : P->reduce_s[h];
       Proc& pn = mode == 0 ? P->node[h] : P->node_s[h];
       Proc& pb = mode == 0 ? P->bcast[h] : P->bcast_s[h];
-      B.gemm(pr, CAT_REDUCE, F64, wt, F64, wt, true, 1, red, 2048);
-      B.gemm(pn, CAT_NODE_GEMM, F64, F64, F64, F64, false, 0, ng1);
+      const Cat cr = mode == 0 ? CAT_REDUCE : CAT_SOLVE_REDUCE;
+      const Cat cg = mode == 0 ? CAT_NODE_GEMM : CAT_SOLVE_NODE;
+      const Cat ct = mode == 0 ? CAT_NODE_TRSM : CAT_SOLVE_NODE;
+      const Cat cb = mode == 0 ? CAT_BCAST : CAT_SOLVE_BCAST;
+      B.gemm(pr, cr, F64, wt, F64, wt, true, 1, red, 2048);
+      B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng1);
       if (mode == 0) {
         B.copy(pn, CAT_NODE_LU, F64, F64, sform);
         B.lu(pn, CAT_NODE_LU, CAT_NODE_LU, CAT_NODE_LU, CAT_NODE_LU, F64, slu);
       }
-      B.trsm(pn, CAT_NODE_TRSM, F64, F64, F64, str);
-      B.gemm(pn, CAT_NODE_GEMM, F64, F64, F64, F64, false, 0, ng2);
-      B.gemm(pb, CAT_BCAST, wt, F64, wt, wt, false, 2, bc);
+      B.trsm(pn, ct, F64, F64, F64, str);
+      B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng2);
+      B.gemm(pb, cb, wt, F64, wt, wt, false, 2, bc);
     }
   }
 

@@ -18,6 +18,7 @@ from ._lib import PlanInfo, SolveReport, System, check, device_lib
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PROFILE_NCAT = 20   # HODLR_PROFILE_NCAT
 K_PACKED = 0
 K_STACKED = 1
 
@@ -243,16 +244,14 @@ class Plan:
     def use_graph(self, enable=True):
         check(self.lib.hodlr_use_graph(self.h, int(enable)), "hodlr_use_graph")
 
-    CATEGORIES = None
-
     def categories(self):
-        return [self.lib.hodlr_profile_category(i).decode() for i in range(16)]
+        return [self.lib.hodlr_profile_category(i).decode() for i in range(PROFILE_NCAT)]
 
     def profile(self, categories=None):
         """Time the launches of the named categories (None = all, [] = off)."""
         names = self.categories()
         if categories is None:
-            mask = (1 << 16) - 1
+            mask = (1 << PROFILE_NCAT) - 1
         else:
             mask = 0
             for c in categories:
@@ -263,7 +262,7 @@ class Plan:
         check(self.lib.hodlr_profile_reset(self.h))
 
     def profile_read(self):
-        n = 16
+        n = PROFILE_NCAT
         t = np.zeros(n)
         c = np.zeros(n, np.int32)
         f = np.zeros(n)

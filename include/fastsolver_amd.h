@@ -183,7 +183,7 @@ int hodlr_use_graph(hodlr_plan_t plan, int32_t enable);
  * events recorded on the plan's stream, without synchronising; read/reset
  * harvest them.  Graph replay is off while a mask is set.  Also reports the
  * algorithmic flops and bytes of the timed launches. */
-#define HODLR_PROFILE_NCAT 16
+#define HODLR_PROFILE_NCAT 20
 int hodlr_profile_enable(hodlr_plan_t plan, int32_t mask);
 int hodlr_profile_reset(hodlr_plan_t plan);
 int hodlr_profile_read(hodlr_plan_t plan, double* times_ms, int32_t* launches, double* flops, double* bytes);
