@@ -58,3 +58,23 @@ def test_dgesv_batched_singular_info():
     dA, dB = hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(np.ones((8, 1)))
     info = hodlr.dgesv_batched([dA], [dB])
     assert info[0] == 2   # LAPACK: U(2,2) is exactly zero
+
+
+def test_integer_known_answers_fp64_and_fp32():
+    """Raw circulant factors (integer entries) through the reduce (t,n) and
+    broadcast (n,n; alpha -1, beta 1) products: every partial sum is an
+    integer below 2^24, so FP64 and FP32 results must equal the fixture bitwise."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "circulant_int_gemm.npz"))
+    A, B, W, eta, C, D = (np.asfortranarray(d[k].astype(np.float64)) for k in ("A", "B", "W", "eta", "C", "D"))
+    dW = hodlr.DeviceArray(W.shape[0], W.shape[1])
+    dC = hodlr.DeviceArray.from_host(C)
+    hodlr.gemm_batched([("t", 1.0, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(B), 0.0, dW),
+                        ("n", -1.0, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(eta), 1.0, dC)])
+    assert np.array_equal(dW.download(), W)
+    assert np.array_equal(dC.download(), D)
+    f = lambda x: np.asfortranarray(x.astype(np.float32))
+    sC = hodlr.DeviceArray.from_host(f(C))
+    hodlr.gemm_batched([("n", -1.0, hodlr.DeviceArray.from_host(f(A)), hodlr.DeviceArray.from_host(f(eta)), 1.0, sC)],
+                       dtype=np.float32)
+    assert np.array_equal(sC.download().astype(np.float64), D)

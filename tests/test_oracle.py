@@ -178,3 +178,17 @@ def test_rel_l2_is_direct_solve_metric():
     x = np.asfortranarray([[1.0, 2.0], [3.0, 4.0]])
     y = np.asfortranarray([[1.0, 2.0], [3.0, 5.0]])
     assert oracle.rel_l2(y, x) == pytest.approx(1.0 / np.sqrt(30.0), rel=1e-15)
+
+
+def test_integer_gemm_fixture_is_exact():
+    """The exact-integer known answers: inputs follow the circulant formula of
+    the reference generator (init_matrix_tasks.cc:176-219) and the stored
+    products equal int64 arithmetic (so FP64/FP32 kernels must match bitwise)."""
+    d = np.load(os.path.join(GOLD, "circulant_int_gemm.npz"))
+    A, B, W, eta, C, D = (d[k].astype(np.int64) for k in ("A", "B", "W", "eta", "C", "D"))
+    rows = np.arange(180, 240)
+    assert np.array_equal(A, (rows[:, None] + np.arange(30)[None, :]) % 30)
+    assert np.array_equal(B, (rows[:, None] + np.arange(62)[None, :]) % 30)
+    assert np.array_equal(W, A.T @ B)
+    assert np.array_equal(D, C - A @ eta)
+    assert np.abs(W).max() < 2 ** 24 and np.abs(D).max() < 2 ** 24
