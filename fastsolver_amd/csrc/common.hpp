@@ -38,6 +38,20 @@ template <> struct Mfma<float> {
 
 template <class TO, class TI> __device__ __forceinline__ TO cvt(TI x) { return TO(x); }
 
+// Global-address-space views of device pointers.  Pointers read from
+// descriptor tables are generic; through these casts the compiler emits
+// global_load/global_store instead of flat ones, which are tracked by vmcnt
+// only, so an LDS wait does not also wait for every global load in flight.
+#define HODLR_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ HODLR_GLOBAL T* glob(T* p) { return (HODLR_GLOBAL T*)p; }
+
+// Out-of-range lanes of a staged tile load from here: the value is already
+// zero, so no select (and no wait for the load) follows the load.
+static __device__ double g_zero_page[2];
+template <class T> __device__ __forceinline__ const HODLR_GLOBAL T* zero_page() {
+  return (const HODLR_GLOBAL T*)g_zero_page;
+}
+
 // ------------------------------------------------------------- tile engine
 // C (BM x BN) += A (BM x BK) * B (BK x BN) with both operands staged in LDS
 // k-major: As[k][m], Bs[k][n].  The four waves split the tile 2 x 2; each wave
@@ -68,17 +82,26 @@ struct Tile {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m0 = (wave & 1) * (BM / 2), n0 = (wave >> 1) * (BN / 2);
     const int li = lane & 15, lk = lane >> 4;
+    // fragments of up to 16 k values are read before their MFMAs so that one
+    // LDS wait covers a group of 4 x WM x WN MFMAs
+    constexpr int KG = KS < 16 ? KS : 16;
+    static_assert(KS % KG == 0 && KG % 4 == 0, "k extent");
 #pragma unroll
-    for (int kk = 0; kk < KS; kk += 4) {
-      MT a[WM], b[WN];
+    for (int k0 = 0; k0 < KS; k0 += KG) {
+      MT a[KG / 4][WM], b[KG / 4][WN];
 #pragma unroll
-      for (int i = 0; i < WM; ++i) a[i] = As[(kk + lk) * pa + m0 + i * 16 + li];
+      for (int q = 0; q < KG / 4; ++q) {
 #pragma unroll
-      for (int j = 0; j < WN; ++j) b[j] = Bs[(kk + lk) * pb + n0 + j * 16 + li];
+        for (int i = 0; i < WM; ++i) a[q][i] = As[(k0 + 4 * q + lk) * pa + m0 + i * 16 + li];
 #pragma unroll
-      for (int i = 0; i < WM; ++i)
+        for (int j = 0; j < WN; ++j) b[q][j] = Bs[(k0 + 4 * q + lk) * pb + n0 + j * 16 + li];
+      }
 #pragma unroll
-        for (int j = 0; j < WN; ++j) acc[i][j] = Mfma<MT>::mma(a[i], b[j], acc[i][j]);
+      for (int q = 0; q < KG / 4; ++q)
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+          for (int j = 0; j < WN; ++j) acc[i][j] = Mfma<MT>::mma(a[q][i], b[q][j], acc[i][j]);
     }
   }
 
@@ -100,30 +123,36 @@ struct Tile {
 // tile (k [k0, k0+BK) x cols [c0, c0+BN)); out-of-range entries are zero.
 //   A: TRANS = false -> A(m, k) at A[m + k*lda];  TRANS = true -> A[k + m*lda]
 //   B: B(k, n) at B[k + n*ldb]
-template <class MT, int BM, int BN, int BK, bool TRANS>
+// Loads are unconditional (an out-of-range lane reads the zero page) and keep
+// the storage type: conversion to the math type happens at the LDS store,
+// after the MFMA work of the current stage, so no wait sits between a load and
+// the MFMAs that run while it is in flight.
+template <class MT, class TA, class TB, int BM, int BN, int BK, bool TRANS>
 struct Stager {
   static constexpr int NA = (BM * BK) / kThreads, NB = (BN * BK) / kThreads;
   static constexpr int PA = BM + 1, PB = BN + 1;
-  MT ra[NA], rb[NB];
+  TA ra[NA];
+  TB rb[NB];
 
-  template <class TA>
-  __device__ __forceinline__ void load_a(const TA* A, int64_t lda, int r0, int mlim, int k0, int klim) {
+  __device__ __forceinline__ void load_a(const HODLR_GLOBAL TA* A, int64_t lda, int r0, int mlim, int k0, int klim) {
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
       const int gm = r0 + m, gk = k0 + k;
-      ra[s] = (gm < mlim && gk < klim) ? MT(TRANS ? A[gk + int64_t(gm) * lda] : A[gm + int64_t(gk) * lda]) : MT(0);
+      const bool ok = gm < mlim && gk < klim;
+      const int64_t off = TRANS ? gk + int64_t(gm) * lda : gm + int64_t(gk) * lda;
+      ra[s] = *(ok ? A + off : zero_page<TA>());
     }
   }
-  template <class TB>
-  __device__ __forceinline__ void load_b(const TB* B, int64_t ldb, int k0, int klim, int c0, int nlim) {
+  __device__ __forceinline__ void load_b(const HODLR_GLOBAL TB* B, int64_t ldb, int k0, int klim, int c0, int nlim) {
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int k = e % BK, n = e / BK;
       const int gk = k0 + k, gn = c0 + n;
-      rb[s] = (gk < klim && gn < nlim) ? MT(B[gk + int64_t(gn) * ldb]) : MT(0);
+      const bool ok = gk < klim && gn < nlim;
+      rb[s] = *(ok ? B + gk + int64_t(gn) * ldb : zero_page<TB>());
     }
   }
   __device__ __forceinline__ void store(MT* As, MT* Bs) const {
@@ -131,12 +160,12 @@ struct Stager {
     for (int s = 0; s < NA; ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
-      As[k * PA + m] = ra[s];
+      As[k * PA + m] = MT(ra[s]);
     }
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
       const int e = threadIdx.x + s * kThreads;
-      Bs[(e % BK) * PB + e / BK] = rb[s];
+      Bs[(e % BK) * PB + e / BK] = MT(rb[s]);
     }
   }
 };
@@ -144,11 +173,12 @@ struct Stager {
 // acc += op(A)[r0.., kb..ke) * B[kb..ke), c0..) with register prefetch and two
 // LDS stages: global loads of step t+1 are in flight while step t computes.
 template <class MT, int BM, int BN, int BK, bool TRANS, class TA, class TB>
-__device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, const TA* A, int64_t lda, int r0,
-                                          int mlim, const TB* B, int64_t ldb, int c0, int nlim, int kb, int ke) {
+__device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, const HODLR_GLOBAL TA* A, int64_t lda,
+                                          int r0, int mlim, const HODLR_GLOBAL TB* B, int64_t ldb, int c0, int nlim,
+                                          int kb, int ke) {
   typedef Tile<MT, BM, BN, BK> T;
   if (kb >= ke) return;
-  Stager<MT, BM, BN, BK, TRANS> st;
+  Stager<MT, TA, TB, BM, BN, BK, TRANS> st;
   st.load_a(A, lda, r0, mlim, kb, ke);
   st.load_b(B, ldb, kb, ke, c0, nlim);
   st.store(lds, lds + BK * T::PA);

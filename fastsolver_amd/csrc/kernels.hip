@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restri
   constexpr bool PREFETCH = CPT <= 16;
   const bool use_beta = d.nsplit == 1 && d.beta != 0.0;
   const int m = d.m, n = d.n;
-  const TC* cin = (const TC*)d.Cin;
+  const HODLR_GLOBAL TC* cin = glob((const TC*)d.Cin);
   const int64_t ldi = d.ldcin;
   TC cpre[PREFETCH ? CPT : 1];
   if (PREFETCH && use_beta) {
@@ -54,20 +54,21 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restri
     for (int s = 0; s < (PREFETCH ? CPT : 1); ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int gr = r0 + e % BM, gc = c0 + e / BM;
-      cpre[s] = (gr < m && gc < n) ? cin[gr + gc * ldi] : TC(0);
+      cpre[s] = *(gr < m && gc < n ? cin + gr + gc * ldi : zero_page<TC>());
     }
   }
 
   TileT t;
   t.zero();
-  gemm_loop<MT, BM, BN, BK, TRANS>(t, lds, (const TA*)d.A, d.lda, r0, d.m, (const TB*)d.B, d.ldb, c0, d.n, kb, ke);
+  gemm_loop<MT, BM, BN, BK, TRANS>(t, lds, glob((const TA*)d.A), d.lda, r0, d.m, glob((const TB*)d.B), d.ldb, c0, d.n,
+                                   kb, ke);
 
   // epilogue through LDS: CH columns per pass, written column by column
   constexpr int CH = (BM * (BN + 1) <= TileT::LDS_ELEMS) ? BN : 32;
   constexpr int PC = BM + 1;
   static_assert(CH * PC <= TileT::LDS_ELEMS, "epilogue chunk must fit the LDS stages");
   const double alpha = d.alpha, beta = d.beta;
-  TC* out = (TC*)d.Cout + (d.nsplit > 1 ? int64_t(split) * d.split_stride : 0);
+  HODLR_GLOBAL TC* out = glob((TC*)d.Cout + (d.nsplit > 1 ? int64_t(split) * d.split_stride : 0));
   const int64_t ldo = d.ldcout;
   MT* cs = lds;
 #pragma unroll
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict
   const SumDesc d = descs[find_desc(descs, ndesc, bid)];
   const int64_t base = (bid - d.tile_begin) * kSumTile;
   const int64_t total = int64_t(d.m) * d.n;
-  const double* P = (const double*)d.P;
-  double* out = (double*)d.out;
+  const HODLR_GLOBAL double* P = glob((const double*)d.P);
+  HODLR_GLOBAL double* out = glob((double*)d.out);
   for (int e = threadIdx.x; e < kSumTile; e += kThreads) {
     int64_t idx = base + e;
     if (idx >= total) break;
@@ -119,8 +120,8 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
   const CopyDesc d = descs[find_desc(descs, ndesc, bid)];
   const int64_t base = (bid - d.tile_begin) * kCopyTile;
   const int64_t total = int64_t(d.rows) * d.cols;
-  const TS* src = (const TS*)d.src;
-  TD* dst = (TD*)d.dst;
+  const HODLR_GLOBAL TS* src = glob((const TS*)d.src);
+  HODLR_GLOBAL TD* dst = glob((TD*)d.dst);
   for (int e = threadIdx.x; e < kCopyTile; e += kThreads) {
     int64_t idx = base + e;
     if (idx >= total) break;
@@ -134,13 +135,12 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 
 // ========================================================================= LU
 // Panel of kLuNB columns starting at k0, factored as kLuNB/PW sub-panels of PW
-// columns held in registers: 1024 threads, thread t owns row ks + t of the
-// sub-panel (blocks of up to 1024 rows).  PW = 16 for fp64 and 32 for fp32
+// columns held in registers: NT threads (256, 512 or 1024, the smallest that
+// covers the batch's largest block), thread t owns row ks + t of the sub-panel.  PW = 16 for fp64 and 32 for fp32
 // keeps the register panel within the 128 VGPRs a 1024-thread group allows.
 // Partial pivoting picks the first largest |a| (idamax); the column below a
 // zero pivot is left unscaled and info records it, as dgetf2 does.
-constexpr int kPanelThreads = 1024;
-constexpr int kPanelWaves = kPanelThreads / 64;
+constexpr int kPanelThreads = 1024;   // largest block the panel kernel holds
 
 template <class T> struct PanelWidth { static constexpr int value = 16; };
 template <> struct PanelWidth<float> { static constexpr int value = 32; };
@@ -150,24 +150,25 @@ __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
   if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
 }
 
-template <class T>
-__device__ __forceinline__ void swap_rows_in_column(T* A, int64_t col, int r1, int r2) {
+template <class PT>
+__device__ __forceinline__ void swap_rows_in_column(PT A, int64_t col, int r1, int r2) {
   if (r1 != r2) {
-    T x = A[r1 + col];
+    const auto x = A[r1 + col];
     A[r1 + col] = A[r2 + col];
     A[r2 + col] = x;
   }
 }
 
-template <class T>
-__global__ __launch_bounds__(kPanelThreads) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+  constexpr int kPanelWaves = NT / 64;
   constexpr int PW = PanelWidth<T>::value;
   constexpr int NSP = kLuNB / PW;
   constexpr int PX = kLuNB + 1;
   const LuDesc d = descs[blockIdx.x];
   const int m = d.m;
   if (k0 >= m) return;
-  T* A = (T*)d.A;
+  HODLR_GLOBAL T* A = glob((T*)d.A);
   const int64_t lda = d.lda;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nb = min(kLuNB, m - k0);
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(kPanelThreads) void lu_panel_kernel(const LuDesc* _
     }
     __syncthreads();
     if (nright > 0) {
-      for (int e = tid; e < PW * nright; e += kPanelThreads) {
+      for (int e = tid; e < PW * nright; e += NT) {
         const int r = e % PW, c = e / PW;
         s_x[r * PX + c] = A[(ks + r) + int64_t(ks + PW + c) * lda];
       }
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(kPanelThreads) void lu_panel_kernel(const LuDesc* _
         }
       }
       __syncthreads();
-      for (int e = tid; e < PW * nright; e += kPanelThreads) {
+      for (int e = tid; e < PW * nright; e += NT) {
         const int r = e % PW, c = e / PW;
         A[(ks + r) + int64_t(ks + PW + c) * lda] = s_x[r * PX + c];
       }
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kPanelThreads) void lu_panel_kernel(const LuDesc* _
   for (int j = 0; j < nb; ++j) any |= s_piv[j] != k0 + j;
   if (any) {
     const int nout = m - nb;
-    for (int c = tid; c < nout; c += kPanelThreads) {
+    for (int c = tid; c < nout; c += NT) {
       const int col = c < k0 ? c : c + nb;
       const int64_t cb = int64_t(col) * lda;
       for (int j = 0; j < nb; ++j) swap_rows_in_column(A, cb, k0 + j, s_piv[j]);
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void lu_rowblock_kernel(const LuDesc* __r
   const int c0 = k0 + nb + blockIdx.x * 64;
   if (c0 >= m) return;
   const int nc = min(64, m - c0);
-  T* A = (T*)d.A;
+  HODLR_GLOBAL T* A = glob((T*)d.A);
   const int64_t lda = d.lda;
   constexpr int P = 65;
   __shared__ T s_l[kLuNB * P];
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restr
   const int r0 = b * kDB;
   if (r0 >= d.m) return;
   const int nr = min(kDB, d.m - r0);
-  const T* A = (const T*)d.A;
+  const HODLR_GLOBAL T* A = glob((const T*)d.A);
   constexpr int P = kDB + 1;
   __shared__ T s_a[kDB * P];   // the diagonal block, s_a[i*P + k] = A(r0+i, r0+k)
   __shared__ T s_l[kDB * P];   // L^-1, s_l[i*P + j]
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restr
     }
   }
   __syncthreads();
-  T* out = (T*)d.dinv + int64_t(b) * 2 * kDB * kDB;
+  HODLR_GLOBAL T* out = glob((T*)d.dinv + int64_t(b) * 2 * kDB * kDB);
   for (int e = threadIdx.x; e < kDB * kDB; e += kThreads) {
     const int i = e % kDB, j = e / kDB;
     out[e] = s_l[i * P + j];
@@ -458,10 +459,11 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restri
   const int c0 = int(bid - d.tile_begin) * CB;
   const int m = d.m, ncols = d.ncols;
   const int nc = min(CB, ncols - c0);
-  const TLU* LU = (const TLU*)d.LU;
-  const TLU* DI = (const TLU*)d.dinv;
-  const TIN* Xin = (const TIN*)d.Xin;
-  TOUT* Xout = (TOUT*)d.Xout;
+  const HODLR_GLOBAL TLU* LU = glob((const TLU*)d.LU);
+  const HODLR_GLOBAL TLU* DI = glob((const TLU*)d.dinv);
+  const HODLR_GLOBAL TIN* Xin = glob((const TIN*)d.Xin);
+  HODLR_GLOBAL TOUT* Xout = glob((TOUT*)d.Xout);
+  const HODLR_GLOBAL int32_t* perm = d.perm ? glob(d.perm) : nullptr;
   const int64_t ldlu = d.ldlu, ldin = d.ldin, ldout = d.ldout;
   const int nblk = (m + RB - 1) / RB;
   TileT t;
@@ -479,7 +481,7 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restri
         MT x = 0;
         if (r < nr && c < nc) {
           if (pass == 0) {
-            const int src = d.perm ? d.perm[r0 + r] : r0 + r;
+            const int src = perm ? perm[r0 + r] : r0 + r;
             x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
           } else {
             x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]) - v;
@@ -487,7 +489,7 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restri
         }
         s_t[r * P + c] = x;
       });
-      const TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
+      const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
       for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
         const int i = e % RB, k = e / RB;
         s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
@@ -523,10 +525,11 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
   const int c0 = int(bid - d.tile_begin) * CB;
   const int m = d.m, ncols = d.ncols;
   const int nc = min(CB, ncols - c0);
-  const TLU* LU = (const TLU*)d.LU;
-  const TLU* DI = (const TLU*)d.dinv;
-  const TIN* Xin = (const TIN*)d.Xin;
-  TOUT* Xout = (TOUT*)d.Xout;
+  const HODLR_GLOBAL TLU* LU = glob((const TLU*)d.LU);
+  const HODLR_GLOBAL TLU* DI = glob((const TLU*)d.dinv);
+  const HODLR_GLOBAL TIN* Xin = glob((const TIN*)d.Xin);
+  HODLR_GLOBAL TOUT* Xout = glob((TOUT*)d.Xout);
+  const HODLR_GLOBAL int32_t* perm = d.perm ? glob(d.perm) : nullptr;
   const int64_t ldlu = d.ldlu, ldin = d.ldin, ldout = d.ldout;
   const int nstep = (m + RB - 1) / RB;
 
@@ -538,7 +541,7 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
     }
   };
   auto load_inv = [&](int b, int which) {
-    const TLU* inv = DI + int64_t(b) * 2 * HB * HB + which * HB * HB;
+    const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * HB * HB + which * HB * HB;
     for (int e = threadIdx.x; e < HB * HB; e += kThreads) s_u[(e / HB) * PI + e % HB] = inv[e];
   };
   // rows [h*64, h*64+64) of s_t <- inv * rows of s_t; also written to Xout
@@ -578,7 +581,7 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
           MT x = 0;
           if (r < nr && c < nc) {
             if (pass == 0) {
-              const int src = d.perm ? d.perm[R0 + r] : R0 + r;
+              const int src = perm ? perm[R0 + r] : R0 + r;
               x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
             } else {
               x = MT(Xout[(R0 + r) + int64_t(c0 + c) * ldout]) - v;
@@ -624,7 +627,8 @@ __global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restri
   const int64_t chunk = bid - d.tile_begin;
   const int64_t b = chunk * kNormChunk, e = min(d.n, b + kNormChunk);
   double s = 0.0;
-  for (int64_t i = b + threadIdx.x; i < e; i += kThreads) s += d.x[i] * d.x[i];
+  const HODLR_GLOBAL double* x = glob(d.x);
+  for (int64_t i = b + threadIdx.x; i < e; i += kThreads) s += x[i] * x[i];
   __shared__ double red[kThreads];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -655,32 +659,29 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   dim3 g(grid1(tiles)), b(kThreads);
 #define HODLR_GEMM(R, A, B, C, M, T, BM, BN) \
   hipLaunchKernelGGL((gemm_kernel<R, A, B, C, M, T, BM, BN>), g, b, 0, s, d, n)
-  const bool big = tile == 1;
+  // one of the three tilings: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32
+#define HODLR_GEMM3(R, A, B, C, M, T)                                  \
+  do {                                                                 \
+    if (tile == 1) HODLR_GEMM(R, A, B, C, M, T, 128, 128);             \
+    else if (tile == 2) HODLR_GEMM(R, A, B, C, M, T, 64, kGemmNarrowBN); \
+    else HODLR_GEMM(R, A, B, C, M, T, 64, 64);                         \
+  } while (0)
   if (ta == F64 && tb == F64 && tc == F64 && mt == F64) {
-    if (trans) {
-      if (big) HODLR_GEMM(1, double, double, double, double, true, 128, 128);
-      else HODLR_GEMM(1, double, double, double, double, true, 64, 64);
-    } else if (role == 2) {
-      if (big) HODLR_GEMM(2, double, double, double, double, false, 128, 128);
-      else HODLR_GEMM(2, double, double, double, double, false, 64, 64);
-    } else if (role == 3) {
-      if (big) HODLR_GEMM(3, double, double, double, double, false, 128, 128);
-      else HODLR_GEMM(3, double, double, double, double, false, 64, 64);
-    } else {
-      HODLR_GEMM(0, double, double, double, double, false, 64, 64);
-    }
+    if (trans) HODLR_GEMM3(1, double, double, double, double, true);
+    else if (role == 2) HODLR_GEMM3(2, double, double, double, double, false);
+    else if (role == 3) HODLR_GEMM3(3, double, double, double, double, false);
+    else if (tile == 2) HODLR_GEMM(0, double, double, double, double, false, 64, kGemmNarrowBN);
+    else HODLR_GEMM(0, double, double, double, double, false, 64, 64);
   } else if (ta == F64 && tb == F32 && tc == F64 && mt == F32 && trans) {
-    if (big) HODLR_GEMM(1, double, float, double, float, true, 128, 128);
-    else HODLR_GEMM(1, double, float, double, float, true, 64, 64);
+    HODLR_GEMM3(1, double, float, double, float, true);
   } else if (ta == F32 && tb == F64 && tc == F32 && mt == F32 && !trans) {
-    if (big) HODLR_GEMM(2, float, double, float, float, false, 128, 128);
-    else HODLR_GEMM(2, float, double, float, float, false, 64, 64);
+    HODLR_GEMM3(2, float, double, float, float, false);
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
-    if (big) HODLR_GEMM(3, float, float, float, float, false, 128, 128);
-    else HODLR_GEMM(3, float, float, float, float, false, 64, 64);
+    HODLR_GEMM3(3, float, float, float, float, false);
   } else {
     return hipErrorInvalidValue;
   }
+#undef HODLR_GEMM3
 #undef HODLR_GEMM
   return hipGetLastError();
 }
@@ -705,9 +706,16 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (rows_max > kPanelThreads) return hipErrorInvalidValue;
-  dim3 g(n), b(kPanelThreads);
-  if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double>), g, b, 0, s, d, k0);
-  else hipLaunchKernelGGL((lu_panel_kernel<float>), g, b, 0, s, d, k0);
+  dim3 g(n);
+#define HODLR_PANEL(NT)                                                                    \
+  do {                                                                                     \
+    if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT>), g, dim3(NT), 0, s, d, k0); \
+    else hipLaunchKernelGGL((lu_panel_kernel<float, NT>), g, dim3(NT), 0, s, d, k0);         \
+  } while (0)
+  if (rows_max <= 256) HODLR_PANEL(256);
+  else if (rows_max <= 512) HODLR_PANEL(512);
+  else HODLR_PANEL(1024);
+#undef HODLR_PANEL
   return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@ namespace hodlr {
 enum DT { F64 = 0, F32 = 1 };
 
 constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
+constexpr int kGemmNarrowBN = 32;   // tile 2: 64 x 32 for problems at most 32 columns wide
 constexpr int kTrsmRB = 64, kTrsmCB = 64;
 constexpr int kLuNB = 64;          // outer panel width of the blocked LU
 constexpr int kCopyTile = 1024;    // elements per copy workgroup

@@ -56,10 +56,10 @@ struct Launch {
   DT t[4];
   bool trans = false;
   int role = 0;
-  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128 (default choice)
-  int gid = -1;          // GEMM launches with both tilings: index into the plan's tile choices
-  size_t off_t[2] = {0, 0};
-  int64_t tiles_t[2] = {0, 0};
+  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32 (default choice)
+  int gid = -1;          // GEMM launches with both tilings 0/1: index into the plan's tile choices
+  size_t off_t[3] = {0, 0, 0};
+  int64_t tiles_t[3] = {0, 0, 0};
   size_t off = 0;        // byte offset of the first descriptor in the blob
   int count = 0;
   int64_t tiles = 0;
@@ -183,15 +183,22 @@ struct Builder {
     // tile size: 128 x 128 for the big roles when the problems are large enough
     // and the batch still fills the GPU; launches of roles 1-3 carry both
     // tilings and the plan's autotuner may switch them
+    // launches whose problems are all at most 32 columns wide (the residual
+    // and correction sweeps on c <= 32 right-hand sides) use 64 x 32 tiles
     int tile = 0;
+    bool narrow = true;
     {
       double sm = 0, sn = 0;
       int cnt = 0;
       int64_t big_tiles = 0;
       for (const G& g : gs)
-        if (g.m > 0 && g.n > 0) { sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128); }
+        if (g.m > 0 && g.n > 0) {
+          sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128);
+          narrow = narrow && g.n <= kGemmNarrowBN;
+        }
       if (role >= 1 && mt == F32 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0))
         tile = 1;
+      if (narrow) tile = 2;
     }
     double work = 0;   // 64-tile rows of K work (independent of the tiling)
     for (const G& g : gs)
@@ -245,8 +252,9 @@ struct Builder {
     L.trans = trans; L.role = role; L.tile = tile;
     L.count = int(d.size());
     L.flops = flops; L.bytes = bytes;
-    for (int cfg = 0; cfg < (role >= 1 ? 2 : 1); ++cfg) {
-      const int TM = cfg ? 128 : kGemmBM, TN = cfg ? 128 : kGemmBN;
+    for (int cfg = 0; cfg < 3; ++cfg) {
+      if (narrow ? cfg != 2 : (cfg == 2 || (cfg == 1 && role == 0))) continue;
+      const int TM = cfg == 1 ? 128 : kGemmBM, TN = cfg == 1 ? 128 : (cfg == 2 ? kGemmNarrowBN : kGemmBN);
       int64_t tiles = 0;
       for (auto& x : d) {
         x.tiles_m = int(cdiv(x.m, TM));
@@ -257,7 +265,7 @@ struct Builder {
       L.off_t[cfg] = blob.add(d);
       L.tiles_t[cfg] = tiles;
     }
-    if (role >= 1 && tunable) L.gid = int(ngid++);
+    if (role >= 1 && tunable && !narrow) L.gid = int(ngid++);
     L.off = L.off_t[tile];
     L.tiles = L.tiles_t[tile];
     pr.l.push_back(L);
