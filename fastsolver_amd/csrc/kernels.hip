@@ -23,8 +23,8 @@ namespace hodlr {
 // ROLE only separates the symbol names of the callers in profiles:
 // 0 node/matvec products, 1 reduce V^T X, 2 broadcast d -= u eta, 3 LU trailing update.
 template <int ROLE, class TA, class TB, class TC, class MT, bool TRANS, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
-  constexpr int BK = kGemmBK;
+__global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
+  constexpr int BK = BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32);
   typedef Tile<MT, BM, BN, BK> TileT;
   __shared__ MT lds[TileT::LDS_ELEMS];
 
@@ -445,9 +445,9 @@ __global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restr
 // diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
 // serial substitution.
 template <class TLU, class TIN, class TOUT>
-__global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
   typedef TLU MT;
-  constexpr int BK = kGemmBK, RB = kTrsmRB, CB = kTrsmCB, P = RB + 1;
+  constexpr int BK = sizeof(MT) == 8 ? kTrsmBK : kTrsmBK32, RB = kTrsmRB, CB = kTrsmCB, P = RB + 1;
   static_assert(RB == kDB, "diagonal blocks and TRSM steps share the 64-row blocking");
   typedef Tile<MT, RB, CB, BK> TileT;
   constexpr int UNION = (TileT::LDS_ELEMS > RB * P) ? TileT::LDS_ELEMS : RB * P;
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmDesc* __restri
 template <class TLU, class TIN, class TOUT, int CB>
 __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
   typedef TLU MT;
-  constexpr int BK = kGemmBK, RB = 128, HB = kDB, PT = CB + 1, PI = HB + 1;
+  constexpr int BK = kGemmBigBK, RB = 128, HB = kDB, PT = CB + 1, PI = HB + 1;
   typedef Tile<MT, RB, CB, BK> Big;
   typedef Tile<MT, HB, CB, BK> Half;
   constexpr int SU = (Big::LDS_ELEMS > HB * PI) ? Big::LDS_ELEMS : HB * PI;
@@ -677,7 +677,8 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   } else if (ta == F32 && tb == F64 && tc == F32 && mt == F32 && !trans) {
     HODLR_GEMM3(2, float, double, float, float, false);
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
-    HODLR_GEMM3(3, float, float, float, float, false);
+    if (role == 2) HODLR_GEMM3(2, float, float, float, float, false);
+    else HODLR_GEMM3(3, float, float, float, float, false);
   } else {
     return hipErrorInvalidValue;
   }

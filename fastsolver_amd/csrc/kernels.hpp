@@ -9,6 +9,9 @@ namespace hodlr {
 enum DT { F64 = 0, F32 = 1 };
 
 constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
+constexpr int kGemmBK32 = 16;       // k step of the 64-wide tiles in fp32
+constexpr int kGemmBigBK = 16;      // k step of the 128 x 128 tiles and of the 128-row TRSM
+constexpr int kTrsmBK = 16, kTrsmBK32 = 16;   // k step of the 64-row TRSM (fp64, fp32)
 constexpr int kGemmNarrowBN = 32;   // tile 2: 64 x 32 for problems at most 32 columns wide
 constexpr int kTrsmRB = 64, kTrsmCB = 64;
 constexpr int kLuNB = 64;          // outer panel width of the blocked LU

@@ -93,6 +93,7 @@ struct SysState {
   std::vector<size_t> z_off;       // per leaf (doubles)
   std::vector<size_t> vtd, vtu, wmv;   // per node (doubles): reduce outputs
   std::vector<size_t> sbuf, eta, spiv, sperm, sdinv;  // per interior node
+  std::vector<size_t> etaf;        // per interior node, FP32 plans: eta0 | eta1 in the workspace type
   std::vector<int> sinfo;          // per interior node: index into node info array
   std::vector<size_t> kofs;        // per leaf (elements of K)
   size_t bnorm = 0, rnorm = 0;     // doubles, c each
@@ -449,6 +450,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     const size_t nn = t.nodes.size();
     S.vtd.assign(nn, 0); S.vtu.assign(nn, 0); S.wmv.assign(nn, 0);
     S.sbuf.assign(nn, 0); S.eta.assign(nn, 0); S.spiv.assign(nn, 0); S.sperm.assign(nn, 0); S.sdinv.assign(nn, 0);
+    S.etaf.assign(nn, 0);
     S.sinfo.assign(nn, -1);
     for (size_t i = 1; i < nn; ++i) {
       const TreeNode& x = t.nodes[i];
@@ -465,6 +467,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
       const int64_t nd = t.col_beg(x.depth + 1);
       S.sbuf[i] = L.take(size_t(r) * r * 8);
       S.eta[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
+      if (we != 8) S.etaf[i] = L.take(2 * size_t(r) * std::max<int64_t>(nd, c) * we);
       S.spiv[i] = L.take(size_t(r) * 4);
       S.sperm[i] = L.take(size_t(r) * 4);
       S.sdinv[i] = L.take(size_t(cdiv(r, 64)) * 2 * 64 * 64 * 8);
@@ -540,7 +543,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
   for (int h = 1; h <= H; ++h) {
     for (int mode = 0; mode < 2; ++mode) {   // 0: fused factor+solve, 1: solve-only
       std::vector<Builder::G> red, ng1, ng2, bc;
-      std::vector<CopyDesc> sform;
+      std::vector<CopyDesc> sform, eround;
       std::vector<LuDesc> slu;
       std::vector<TrsmDesc> str;
       for (SysState& S : P->sys) {
@@ -606,8 +609,20 @@ void build_all(hodlr_plan_s* P, Builder& B) {
           b.A = vtu[0]; b.lda = r; b.B = eta0; b.ldb = r; b.Cin = vtd[0]; b.ldcin = r; b.Cout = vtd[0]; b.ldcout = r;
           b.m = r; b.n = nd; b.k = r; b.alpha = -1.0; b.beta = 1.0;
           ng2.push_back(b);
-          // broadcast: d0 -= u0 eta0, d1 -= u1 eta1 (eta1 in the VTd0 slot)
-          const double* etas[2] = {eta0, vtd[0]};
+          // broadcast: d0 -= u0 eta0, d1 -= u1 eta1 (eta1 in the VTd0 slot); an
+          // FP32 plan first rounds both etas to FP32 once, so the broadcast
+          // tiles stream 4-byte operands
+          const void* etas[2] = {eta0, vtd[0]};
+          if (wt != F64) {
+            char* ef = P->ws.p + S.etaf[pi];
+            for (int k = 0; k < 2; ++k) {
+              CopyDesc cd{};
+              cd.src = etas[k]; cd.lds = r; cd.dst = ef + size_t(k) * r * nd * we; cd.ldd = r;
+              cd.rows = r; cd.cols = nd; cd.alpha = 1.0;
+              eround.push_back(cd);
+              etas[k] = cd.dst;
+            }
+          }
           for (int k = 0; k < 2; ++k) {
             const TreeNode& C = t.nodes[ch[k]];
             Builder::G g{};
@@ -634,7 +649,8 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       }
       B.trsm(pn, ct, F64, F64, F64, str);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng2);
-      B.gemm(pb, cb, wt, F64, wt, wt, false, 2, bc);
+      B.copy(pn, cg, F64, wt, eround);
+      B.gemm(pb, cb, wt, wt == F64 ? F64 : wt, wt, wt, false, 2, bc);
     }
   }
 
