@@ -38,9 +38,10 @@ static int run(int argc, char** argv) {
   fs.display_launch_time();
   double rr = fs.residual();
   std::printf("Residual: %.3e\n", rr);
-  hMatrix.save_solution();
+  int nregion = nleaf;
+  double err = compute_L2_error(hMatrix, seed, nRow, nregion, nRHS, rank, diagonal);
   std::cout << "================================\n" << std::endl;
-  return rr <= 1e-12 ? 0 : 1;
+  return (rr <= 1e-12 && err <= 1e-12) ? 0 : 1;
 }
 
 int main(int argc, char** argv) {

@@ -11,6 +11,8 @@
 //   launch_level, get_num_rhs, get_num_leaf, get_file_soln   same
 //   FastSolver::bfs_solve(hmat, range, ctx, rt)       bfs_solve(hmat, range) (fast_solver.h:19-20)
 //   register_solver_tasks()                           register_solver_tasks(): checks a GPU is present
+//   compute_L2_error(hmat, seed, rows, nregions,      same minus ctx/rt (include/solver/direct_solve.h:20-24)
+//                    rhs_cols, rank, diag, ctx, rt)
 //
 // The matrix lives in device memory owned by the HodlrMatrix; the solve is
 // not destructive (the reference overwrites K and U, hodlr_matrix.cc / solver_tasks.cc:437).
@@ -18,6 +20,7 @@
 #define FASTSOLVER_AMD_HPP
 
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <cmath>
 #include <fstream>
@@ -249,6 +252,90 @@ class FastSolver {
   double time_ms_ = 0;
   hodlr_solve_report_t report_{};
 };
+
+// The reference's error report for its circulant test matrix
+// (src/solver/direct_solve.cc:128-245): saves the solver's solution,
+// regenerates the right-hand side (drand48 reseeded with `seed` at the start of
+// each of `nregions` row blocks, drawn row by row), solves A = diag*I + U U^T,
+// U(i,j) = (i+j) mod rank, exactly -- by the Woodbury identity in long double,
+// x = (b - U (diag*I + U^T U)^-1 U^T b) / diag, where the reference runs a dense
+// N x N dgesv -- writes it to soln_ref.txt, prints and returns
+// sqrt(sum (x - x_ref)^2 / sum x_ref^2).
+inline double compute_L2_error(const HodlrMatrix& m, long seed, int rhs_rows, int nregions, int rhs_cols,
+                               int rank, double diag) {
+  m.save_solution();
+  const int n = rhs_rows, c = rhs_cols, r = rank;
+  std::vector<double> b(size_t(n) * c);
+  const int block = n / nregions;
+  for (int g = 0; g < nregions; ++g) {
+    drand48_data buf;
+    srand48_r(seed, &buf);
+    for (int i = 0; i < block; ++i)
+      for (int j = 0; j < c; ++j) drand48_r(&buf, &b[size_t(g * block + i) + size_t(j) * n]);
+  }
+  typedef long double LD;
+  auto U = [r](int i, int k) { return LD((i + k) % r); };
+  // G = diag*I + U^T U (r x r), W = U^T b (r x c)
+  std::vector<LD> G(size_t(r) * r, 0), W(size_t(r) * c, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < r; ++k) {
+      const LD u = U(i, k);
+      for (int l = 0; l < r; ++l) G[k + size_t(l) * r] += u * U(i, l);
+      for (int j = 0; j < c; ++j) W[k + size_t(j) * r] += u * LD(b[i + size_t(j) * n]);
+    }
+  for (int k = 0; k < r; ++k) G[k + size_t(k) * r] += diag;
+  // Gaussian elimination with partial pivoting: W <- G^-1 W
+  for (int k = 0; k < r; ++k) {
+    int p = k;
+    for (int i = k + 1; i < r; ++i)
+      if (std::fabs(double(G[i + size_t(k) * r])) > std::fabs(double(G[p + size_t(k) * r]))) p = i;
+    if (p != k) {
+      for (int l = 0; l < r; ++l) std::swap(G[k + size_t(l) * r], G[p + size_t(l) * r]);
+      for (int j = 0; j < c; ++j) std::swap(W[k + size_t(j) * r], W[p + size_t(j) * r]);
+    }
+    for (int i = k + 1; i < r; ++i) {
+      const LD f = G[i + size_t(k) * r] / G[k + size_t(k) * r];
+      for (int l = k; l < r; ++l) G[i + size_t(l) * r] -= f * G[k + size_t(l) * r];
+      for (int j = 0; j < c; ++j) W[i + size_t(j) * r] -= f * W[k + size_t(j) * r];
+    }
+  }
+  for (int j = 0; j < c; ++j)
+    for (int k = r - 1; k >= 0; --k) {
+      LD x = W[k + size_t(j) * r];
+      for (int l = k + 1; l < r; ++l) x -= G[k + size_t(l) * r] * W[l + size_t(j) * r];
+      W[k + size_t(j) * r] = x / G[k + size_t(k) * r];
+    }
+  std::vector<double> xref(size_t(n) * c);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < c; ++j) {
+      LD acc = b[i + size_t(j) * n];
+      for (int k = 0; k < r; ++k) acc -= U(i, k) * W[k + size_t(j) * r];
+      xref[i + size_t(j) * n] = double(acc / diag);
+    }
+  {
+    std::ofstream ofs("soln_ref.txt");
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < c; ++j) ofs << std::setprecision(20) << xref[i + size_t(j) * n] << '\t';
+      ofs << std::endl;
+    }
+  }
+  std::vector<double> soln(size_t(n) * c, 0.0);
+  {
+    std::ifstream ifs(m.get_file_soln().c_str());
+    if (!ifs) throw std::runtime_error("compute_L2_error: cannot open " + m.get_file_soln());
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < c; ++j)
+        if (!(ifs >> soln[i + size_t(j) * n])) throw std::runtime_error("compute_L2_error: short solution file");
+  }
+  double diff = 0, denom = 0;
+  for (size_t e = 0; e < soln.size(); ++e) {
+    diff += (soln[e] - xref[e]) * (soln[e] - xref[e]);
+    denom += xref[e] * xref[e];
+  }
+  const double err = std::sqrt(diff / denom);
+  std::cout << "Err: " << err << std::endl;
+  return err;
+}
 
 }  // namespace fastsolver_amd
 

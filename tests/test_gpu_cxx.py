@@ -33,6 +33,10 @@ def test_single_launch_driver(tmp_path, precision):
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0, (p.returncode, p.stderr)
     assert "Residual:" in p.stdout
+    # the reference's own error report (compute_L2_error), computed by the facade
+    err = float(p.stdout.split("Err:")[1].split()[0])
+    assert err <= 1e-12, err
+    assert (tmp_path / "soln_ref.txt").exists()
     spec = gen.single_launch_case()
     X, V, K = host_system(spec)
     b = X[:, :spec.nrhs].copy(order="F")
