@@ -9,6 +9,8 @@ namespace hodlr {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16;
 
 constexpr int kThreads = 256;  // 4 wavefronts of 64 lanes
 
@@ -36,7 +38,21 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <class TO, class TI> __device__ __forceinline__ TO cvt(TI x) { return TO(x); }
+// bf16 16x16x32 (gfx950): lane l feeds A(l%16, 8*(l/16) + e) and
+// B(8*(l/16) + e, l%16), e = 0..7; accumulators as fp32 (row 4*(l/16) + v).
+template <> struct Mfma<bf16> {
+  typedef f32x4 acc_t;
+  typedef float acc_e;
+  static __device__ __forceinline__ acc_t mma(bf16x8 a, bf16x8 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// storage -> math type (bf16 rounds to nearest even through fp32)
+template <class MT, class TI> __device__ __forceinline__ MT to_math(TI x) { return MT(x); }
+template <> __device__ __forceinline__ bf16 to_math<bf16, double>(double x) { return bf16(float(x)); }
+template <> __device__ __forceinline__ bf16 to_math<bf16, float>(float x) { return bf16(x); }
 
 // Global-address-space views of device pointers.  Pointers read from
 // descriptor tables are generic; through these casts the compiler emits
@@ -62,6 +78,7 @@ struct Tile {
   static constexpr int PB = BN + 1;
   static constexpr int WM = BM / 32;
   static constexpr int WN = BN / 32;
+  static constexpr int BOFF = BK * PA;           // B stage offset within a stage
   static constexpr int STAGE = BK * (PA + PB);   // one LDS stage (A then B)
   static constexpr int LDS_ELEMS = 2 * STAGE;    // double buffered
   typedef typename Mfma<MT>::acc_t acc_t;
@@ -119,6 +136,61 @@ struct Tile {
   }
 };
 
+// bf16 tiles: operands staged "octet-planar", element (k, m) of A at
+// As[((k / 8) * BM + m) * 8 + k % 8] (B likewise with n), so each lane reads
+// its 8 consecutive k of one row with a single 16-byte LDS load per MFMA.
+template <int BM, int BN, int BK>
+struct Tile<bf16, BM, BN, BK> {
+  static_assert(BK % 32 == 0, "bf16 MFMA consumes k in steps of 32");
+  static constexpr int WM = BM / 32;
+  static constexpr int WN = BN / 32;
+  static constexpr int BOFF = BK * BM;
+  static constexpr int STAGE = BK * (BM + BN);
+  static constexpr int LDS_ELEMS = 2 * STAGE;
+  typedef f32x4 acc_t;
+  acc_t acc[WM][WN];
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
+  }
+  template <int KS>
+  __device__ __forceinline__ void mma(const bf16* As, const bf16* Bs) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = (wave & 1) * (BM / 2), n0 = (wave >> 1) * (BN / 2);
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 32) {
+      bf16x8 a[WM], b[WN];
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+        a[i] = *(const bf16x8*)(As + ((kk / 8 + lk) * BM + m0 + i * 16 + li) * 8);
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        b[j] = *(const bf16x8*)(Bs + ((kk / 8 + lk) * BN + n0 + j * 16 + li) * 8);
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = Mfma<bf16>::mma(a[i], b[j], acc[i][j]);
+    }
+  }
+  template <class F> __device__ __forceinline__ void for_each(F&& f) const {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = (wave & 1) * (BM / 2), n0 = (wave >> 1) * (BN / 2);
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          f(m0 + i * 16 + Mfma<bf16>::row(lane, v), n0 + j * 16 + (lane & 15), acc[i][j][v]);
+  }
+};
+
 // Register staging of one A tile (rows [r0, r0+BM) x k [k0, k0+BK)) and one B
 // tile (k [k0, k0+BK) x cols [c0, c0+BN)); out-of-range entries are zero.
 //   A: TRANS = false -> A(m, k) at A[m + k*lda];  TRANS = true -> A[k + m*lda]
@@ -160,12 +232,34 @@ struct Stager {
     for (int s = 0; s < NA; ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
-      As[k * PA + m] = MT(ra[s]);
+      As[k * PA + m] = to_math<MT>(ra[s]);
     }
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
       const int e = threadIdx.x + s * kThreads;
-      Bs[(e % BK) * PB + e / BK] = MT(rb[s]);
+      Bs[(e % BK) * PB + e / BK] = to_math<MT>(rb[s]);
+    }
+  }
+};
+
+// bf16 staging: the same loads, stored octet-planar (see Tile<bf16>)
+template <class TA, class TB, int BM, int BN, int BK, bool TRANS>
+struct Stager<bf16, TA, TB, BM, BN, BK, TRANS> : Stager<float, TA, TB, BM, BN, BK, TRANS> {
+  typedef Stager<float, TA, TB, BM, BN, BK, TRANS> Base;
+  using Base::NA;
+  using Base::NB;
+  __device__ __forceinline__ void store(bf16* As, bf16* Bs) const {
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int m = TRANS ? e / BK : e % BM, k = TRANS ? e % BK : e / BM;
+      As[((k >> 3) * BM + m) * 8 + (k & 7)] = to_math<bf16>(this->ra[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      const int e = threadIdx.x + s * kThreads;
+      const int k = e % BK, n = e / BK;
+      Bs[((k >> 3) * BN + n) * 8 + (k & 7)] = to_math<bf16>(this->rb[s]);
     }
   }
 };
@@ -181,7 +275,7 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
   Stager<MT, TA, TB, BM, BN, BK, TRANS> st;
   st.load_a(A, lda, r0, mlim, kb, ke);
   st.load_b(B, ldb, kb, ke, c0, nlim);
-  st.store(lds, lds + BK * T::PA);
+  st.store(lds, lds + T::BOFF);
   __syncthreads();
   int stage = 0;
   for (int k0 = kb; k0 < ke; k0 += BK) {
@@ -191,10 +285,10 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
       st.load_b(B, ldb, k0 + BK, ke, c0, nlim);
     }
     MT* cur = lds + stage * T::STAGE;
-    t.template mma<BK>(cur, cur + BK * T::PA);
+    t.template mma<BK>(cur, cur + T::BOFF);
     if (more) {
       MT* nxt = lds + (stage ^ 1) * T::STAGE;
-      st.store(nxt, nxt + BK * T::PA);
+      st.store(nxt, nxt + T::BOFF);
     }
     __syncthreads();
     stage ^= 1;

@@ -24,9 +24,17 @@ namespace hodlr {
 // 0 node/matvec products, 1 reduce V^T X, 2 broadcast d -= u eta, 3 LU trailing update.
 template <int ROLE, class TA, class TB, class TC, class MT, bool TRANS, int BM, int BN>
 __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
-  constexpr int BK = BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32);
+  constexpr int BK = sizeof(MT) == 2 ? kGemmBKbf16
+                                    : (BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32));
   typedef Tile<MT, BM, BN, BK> TileT;
-  __shared__ MT lds[TileT::LDS_ELEMS];
+  typedef typename Mfma<MT>::acc_e AE;
+  // the epilogue stages CH columns of the accumulator tile through the same LDS
+  constexpr int STAGE_BYTES = TileT::LDS_ELEMS * int(sizeof(MT));
+  constexpr int CH = (BM * (BN + 1) * int(sizeof(AE)) <= STAGE_BYTES) ? BN : 32;
+  constexpr int PC = BM + 1;
+  constexpr int LDS_BYTES = STAGE_BYTES > CH * PC * int(sizeof(AE)) ? STAGE_BYTES : CH * PC * int(sizeof(AE));
+  __shared__ __attribute__((aligned(16))) char lds_raw[LDS_BYTES];
+  MT* lds = (MT*)lds_raw;
 
   const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const int p = find_desc(descs, ndesc, bid);
@@ -64,28 +72,27 @@ __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __res
                                    kb, ke);
 
   // epilogue through LDS: CH columns per pass, written column by column
-  constexpr int CH = (BM * (BN + 1) <= TileT::LDS_ELEMS) ? BN : 32;
-  constexpr int PC = BM + 1;
-  static_assert(CH * PC <= TileT::LDS_ELEMS, "epilogue chunk must fit the LDS stages");
   const double alpha = d.alpha, beta = d.beta;
   HODLR_GLOBAL TC* out = glob((TC*)d.Cout + (d.nsplit > 1 ? int64_t(split) * d.split_stride : 0));
   const int64_t ldo = d.ldcout;
-  MT* cs = lds;
+  AE* cs = (AE*)lds_raw;
+  constexpr int SPQ = (BM * CH) / kThreads;   // elements per thread and pass
 #pragma unroll
   for (int q = 0; q < BN / CH; ++q) {
-    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+    t.for_each([&](int r, int c, AE v) {
       if (c / CH == q) cs[(c - q * CH) * PC + r] = v;
     });
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < (BM * CH) / kThreads; ++s) {
+    for (int s = 0; s < SPQ; ++s) {
       const int e = threadIdx.x + s * kThreads;
       const int r = e % BM, c = e / BM;
       const int gr = r0 + r, gc = c0 + q * CH + c;
       if (gr < m && gc < n) {
         double x = alpha * double(cs[c * PC + r]);
         if (use_beta) {
-          if constexpr (PREFETCH) x += beta * double(cpre[s]);
+          // cpre holds the tile column-major in kThreads-strided slots: pass q, slot s
+          if constexpr (PREFETCH) x += beta * double(cpre[q * SPQ + s]);
           else x += beta * double(cin[gr + gc * ldi]);
         }
         out[gr + gc * ldo] = TC(x);
@@ -444,14 +451,20 @@ __global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restr
 // solved with a prefetching MFMA product, then multiplies by the inverted
 // diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
 // serial substitution.
-template <class TLU, class TIN, class TOUT>
+template <class TLU, class TIN, class TOUT, class UT>
 __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+  // MT: type of the diagonal-block products and of the stored factor; UT: math
+  // type of the products with the solved rows (MT, or bf16 in the bf16 plans)
   typedef TLU MT;
   constexpr int BK = sizeof(MT) == 8 ? kTrsmBK : kTrsmBK32, RB = kTrsmRB, CB = kTrsmCB, P = RB + 1;
+  constexpr int BKU = sizeof(UT) == 2 ? kGemmBKbf16 : BK;
   static_assert(RB == kDB, "diagonal blocks and TRSM steps share the 64-row blocking");
   typedef Tile<MT, RB, CB, BK> TileT;
-  constexpr int UNION = (TileT::LDS_ELEMS > RB * P) ? TileT::LDS_ELEMS : RB * P;
-  __shared__ MT s_u[UNION];   // MFMA stages, then the inverted diagonal block (As layout)
+  typedef Tile<UT, RB, CB, BKU> TileU;
+  constexpr int UBYTES = TileU::LDS_ELEMS * int(sizeof(UT)), DBYTES = RB * P * int(sizeof(MT));
+  // MFMA stages of the update, then the inverted diagonal block (As layout)
+  __shared__ __attribute__((aligned(16))) char s_u_raw[UBYTES > DBYTES ? UBYTES : DBYTES];
+  MT* s_u = (MT*)s_u_raw;
   __shared__ MT s_t[RB * P];  // right-hand-side block, s_t[r*P + c] (Bs layout)
 
   const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
@@ -467,17 +480,18 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
   const int64_t ldlu = d.ldlu, ldin = d.ldin, ldout = d.ldout;
   const int nblk = (m + RB - 1) / RB;
   TileT t;
+  TileU tu;
 
   for (int pass = 0; pass < 2; ++pass) {        // 0: forward with L, 1: backward with U
     for (int s = 0; s < nblk; ++s) {
       const int b = pass == 0 ? s : nblk - 1 - s;
       const int r0 = b * RB, nr = min(RB, m - r0);
-      t.zero();
+      tu.zero();
       if (pass == 0)
-        gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
+        gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
       else
-        gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
-      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+        gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
+      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) {
         MT x = 0;
         if (r < nr && c < nc) {
           if (pass == 0) {
@@ -679,6 +693,11 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
     if (role == 2) HODLR_GEMM3(2, float, float, float, float, false);
     else HODLR_GEMM3(3, float, float, float, float, false);
+  } else if (ta == F64 && tb == F32 && tc == F64 && mt == BF16 && trans) {
+    HODLR_GEMM3(1, double, float, double, bf16, true);
+  } else if (ta == F32 && tb == F32 && tc == F32 && mt == BF16 && !trans) {
+    if (role == 2) HODLR_GEMM3(2, float, float, float, bf16, false);
+    else HODLR_GEMM3(3, float, float, float, bf16, false);
   } else {
     return hipErrorInvalidValue;
   }
@@ -749,14 +768,22 @@ hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_trsm(DT tlu, DT tin, DT tout, int variant, const TrsmDesc* d, int n, int64_t tiles,
+hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDesc* d, int n, int64_t tiles,
                        hipStream_t s) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
-  if (variant == 0) {
-    if (tlu == F64 && tin == F64 && tout == F64) hipLaunchKernelGGL((trsm_kernel<double, double, double>), g, b, 0, s, d, n);
-    else if (tlu == F32 && tin == F64 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, double, float>), g, b, 0, s, d, n);
-    else if (tlu == F32 && tin == F32 && tout == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float>), g, b, 0, s, d, n);
+  if (ut == BF16) {
+    if (variant != 0 || tlu != F32 || tout != F32) return hipErrorInvalidValue;
+    if (tin == F64) hipLaunchKernelGGL((trsm_kernel<float, double, float, bf16>), g, b, 0, s, d, n);
+    else if (tin == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float, bf16>), g, b, 0, s, d, n);
+    else return hipErrorInvalidValue;
+  } else if (variant == 0) {
+    if (tlu == F64 && tin == F64 && tout == F64)
+      hipLaunchKernelGGL((trsm_kernel<double, double, double, double>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F64 && tout == F32)
+      hipLaunchKernelGGL((trsm_kernel<float, double, float, float>), g, b, 0, s, d, n);
+    else if (tlu == F32 && tin == F32 && tout == F32)
+      hipLaunchKernelGGL((trsm_kernel<float, float, float, float>), g, b, 0, s, d, n);
     else return hipErrorInvalidValue;
   } else {
     if (tlu == F64 && tin == F64 && tout == F64)

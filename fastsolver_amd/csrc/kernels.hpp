@@ -6,11 +6,12 @@
 
 namespace hodlr {
 
-enum DT { F64 = 0, F32 = 1 };
+enum DT { F64 = 0, F32 = 1, BF16 = 2 };   // BF16: math type only (operands stored as F32/F64)
 
 constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
 constexpr int kGemmBK32 = 16;       // k step of the 64-wide tiles in fp32
 constexpr int kGemmBigBK = 16;      // k step of the 128 x 128 tiles and of the 128-row TRSM
+constexpr int kGemmBKbf16 = 32;     // k step of bf16 tiles (one 16x16x32 MFMA)
 constexpr int kTrsmBK = 16, kTrsmBK32 = 16;   // k step of the 64-row TRSM (fp64, fp32)
 constexpr int kGemmNarrowBN = 32;   // tile 2: 64 x 32 for problems at most 32 columns wide
 constexpr int kTrsmRB = 64, kTrsmCB = 64;
@@ -20,7 +21,8 @@ constexpr int kSumTile = 1024;
 constexpr int64_t kNormChunk = 65536;
 
 // Supported (A, B, C, compute) type tuples:
-//   (F64,F64,F64,F64) N and T;  (F64,F32,F64,F32) T;  (F32,F64,F32,F32) N;  (F32,F32,F32,F32) N.
+//   (F64,F64,F64,F64) N and T;  (F64,F32,F64,F32) T;  (F32,F64,F32,F32) N;  (F32,F32,F32,F32) N;
+//   (F64,F32,F64,BF16) T;  (F32,F32,F32,BF16) N.
 // tile 0: 64 x 64 workgroup tiles, 1: 128 x 128 (roles 1-3 only)
 hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int tile, const GemmDesc* d, int n,
                        int64_t tiles, hipStream_t s);
@@ -31,9 +33,10 @@ hipError_t launch_lu_rowblock(DT t, const LuDesc* d, int n, int k0, int mmax, hi
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
 hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s);  // inverted 64x64 diagonal blocks
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
-// (TLU, TIN, TOUT) in {(F64,F64,F64), (F32,F64,F32), (F32,F32,F32)}; compute type = TLU
+// (TLU, TIN, TOUT) in {(F64,F64,F64), (F32,F64,F32), (F32,F32,F32)}; compute type = TLU, except the
+// products with the solved rows, which run in `ut` (= TLU, or BF16 with TLU = F32, variant 0 only)
 // variant 0: 64-row steps, 64-column panels; 1: 128-row steps, trsm_panel_cols(tlu)-column panels
-hipError_t launch_trsm(DT tlu, DT tin, DT tout, int variant, const TrsmDesc* d, int n, int64_t tiles,
+hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDesc* d, int n, int64_t tiles,
                        hipStream_t s);
 inline int trsm_panel_cols(DT tlu, int variant) { return variant == 0 ? kTrsmCB : (tlu == F64 ? 64 : 128); }
 hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s);

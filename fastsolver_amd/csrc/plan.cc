@@ -107,6 +107,7 @@ using namespace hodlr;
 struct hodlr_plan_s {
   int precision = HODLR_PREC_FP64;
   DT wt = F64;                     // workspace element type
+  DT mt = F64;                     // math type of the leaf updates, TRSM updates, reduce and broadcast
   hipStream_t stream = nullptr;
   std::vector<SysState> sys;
   int height = 0, nleaves = 0, nnodes = 0;
@@ -197,7 +198,7 @@ struct Builder {
           sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128);
           narrow = narrow && g.n <= kGemmNarrowBN;
         }
-      if (role >= 1 && mt == F32 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0))
+      if (role >= 1 && mt != F64 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0))
         tile = 1;
       if (narrow) tile = 2;
     }
@@ -299,7 +300,9 @@ struct Builder {
   }
 
   // Blocked LU with partial pivoting of a batch of square blocks.
-  void lu(Proc& pr, Cat panel_cat, Cat row_cat, Cat upd_cat, Cat misc_cat, DT t, const std::vector<LuDesc>& v) {
+  void lu(Proc& pr, Cat panel_cat, Cat row_cat, Cat upd_cat, Cat misc_cat, DT t, const std::vector<LuDesc>& v,
+          DT upd_mt = F64) {
+    if (upd_mt != BF16) upd_mt = t;
     if (v.empty()) return;
     int mmax = 0;
     for (auto& x : v) mmax = std::max(mmax, x.m);
@@ -339,7 +342,7 @@ struct Builder {
           g.alpha = -1.0; g.beta = 1.0;
           gs.push_back(g);
         }
-        gemm(pr, upd_cat, t, t, t, t, false, 3, gs);
+        gemm(pr, upd_cat, t, t, t, upd_mt, false, 3, gs);
       }
     }
     Launch Pm;
@@ -352,7 +355,8 @@ struct Builder {
 
   // Both TRSM variants (64-row steps / 128-row steps) get descriptor sets;
   // the plan's launches are tunable like the GEMM tilings.
-  void trsm(Proc& pr, Cat cat, DT tlu, DT tin, DT tout, std::vector<TrsmDesc> v) {
+  void trsm(Proc& pr, Cat cat, DT tlu, DT tin, DT tout, std::vector<TrsmDesc> v, DT ut = F64) {
+    if (ut != BF16) ut = tlu;
     std::vector<TrsmDesc> d;
     double flops = 0, bytes = 0;
     for (auto& x : v) {
@@ -363,9 +367,10 @@ struct Builder {
     }
     if (d.empty()) return;
     Launch L;
-    L.kind = K_TRSM; L.cat = cat; L.t[0] = tlu; L.t[1] = tin; L.t[2] = tout;
+    L.kind = K_TRSM; L.cat = cat; L.t[0] = tlu; L.t[1] = tin; L.t[2] = tout; L.t[3] = ut;
     L.count = int(d.size()); L.flops = flops; L.bytes = bytes;
-    for (int var = 0; var < 2; ++var) {
+    const bool both = ut != BF16;   // bf16 updates exist for the 64-row variant only
+    for (int var = 0; var < (both ? 2 : 1); ++var) {
       int64_t t = 0;
       for (auto& x : d) {
         x.tile_begin = t;
@@ -374,8 +379,8 @@ struct Builder {
       L.off_t[var] = blob.add(d);
       L.tiles_t[var] = t;
     }
-    L.tile = 1;
-    if (tunable) L.gid = int(ngid++);
+    L.tile = both ? 1 : 0;
+    if (tunable && both) L.gid = int(ngid++);
     L.off = L.off_t[L.tile];
     L.tiles = L.tiles_t[L.tile];
     pr.l.push_back(L);
@@ -534,9 +539,9 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       leaf_base += int(t.leaves.size());
     }
     B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, kc);
-    B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus);
-    B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr);
-    B.trsm(P->solve_only_leaf, CAT_SOLVE_TRSM, wt, F64, wt, trs);
+    B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus, P->mt);
+    B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr, P->mt);
+    B.trsm(P->solve_only_leaf, CAT_SOLVE_TRSM, wt, F64, wt, trs, P->mt);
   }
 
   // ---------------------------------------------------------------- heights
@@ -641,7 +646,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       const Cat cg = mode == 0 ? CAT_NODE_GEMM : CAT_SOLVE_NODE;
       const Cat ct = mode == 0 ? CAT_NODE_TRSM : CAT_SOLVE_NODE;
       const Cat cb = mode == 0 ? CAT_BCAST : CAT_SOLVE_BCAST;
-      B.gemm(pr, cr, F64, wt, F64, wt, true, 1, red, 2048);
+      B.gemm(pr, cr, F64, wt, F64, P->mt, true, 1, red, 2048);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng1);
       if (mode == 0) {
         B.copy(pn, CAT_NODE_LU, F64, F64, sform);
@@ -650,7 +655,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       B.trsm(pn, ct, F64, F64, F64, str);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng2);
       B.copy(pn, cg, F64, wt, eround);
-      B.gemm(pb, cb, wt, wt == F64 ? F64 : wt, wt, wt, false, 2, bc);
+      B.gemm(pb, cb, wt, wt == F64 ? F64 : wt, wt, P->mt, false, 2, bc);
     }
   }
 
@@ -764,7 +769,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
         if (P->tuning) var = P->tune_cfg;
         else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) var = P->tile_choice[L.gid];
       }
-      return launch_trsm(L.t[0], L.t[1], L.t[2], var, (const TrsmDesc*)(P->descs.p + L.off_t[var]), L.count,
+      return launch_trsm(L.t[0], L.t[1], L.t[2], L.t[3], var, (const TrsmDesc*)(P->descs.p + L.off_t[var]), L.count,
                          L.tiles_t[var], s);
     }
     case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s);
@@ -979,11 +984,13 @@ const char* hodlr_profile_category(int32_t i) { return (i >= 0 && i < CAT_N) ? k
 int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t nsys, int32_t precision,
                       void* stream) {
   if (!plan || !systems || nsys <= 0) return HODLR_ERR_ARG;
-  if (precision != HODLR_PREC_FP64 && precision != HODLR_PREC_FP32) return HODLR_ERR_ARG;
+  if (precision != HODLR_PREC_FP64 && precision != HODLR_PREC_FP32 && precision != HODLR_PREC_BF16)
+    return HODLR_ERR_ARG;
   *plan = nullptr;
   std::unique_ptr<hodlr_plan_s> P(new hodlr_plan_s());
   P->precision = precision;
   P->wt = precision == HODLR_PREC_FP64 ? F64 : F32;
+  P->mt = precision == HODLR_PREC_BF16 ? BF16 : P->wt;
   P->stream = (hipStream_t)stream;
   P->sys.resize(nsys);
   for (int s = 0; s < nsys; ++s) {
@@ -1186,7 +1193,7 @@ static int run_oneoff(hodlr_plan_s& P, Builder& B, const Proc& pr) {
   return rc;
 }
 
-static int gemm_batched(DT t, int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
+static int gemm_batched(DT t, DT mt, int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
                         const double* alpha, const void* const* A, const int64_t* lda, const void* const* B,
                         const int64_t* ldb, const double* beta, void* const* C, const int64_t* ldc, void* stream) {
   if (batch <= 0 || !transa || !m || !n || !k || !alpha || !A || !lda || !B || !ldb || !beta || !C || !ldc)
@@ -1212,7 +1219,7 @@ static int gemm_batched(DT t, int32_t batch, const char* transa, const int32_t* 
     }
     if (gs.empty()) continue;
     if (t == F32 && tr) return HODLR_ERR_UNSUPPORTED;   // fp32 products are NN only
-    Bd.gemm(pr, CAT_OTHER, t, t, t, t, tr == 1, tr == 1 ? 1 : (t == F32 ? 3 : 0), gs);
+    Bd.gemm(pr, CAT_OTHER, t, t, t, mt, tr == 1, tr == 1 ? 1 : (t == F32 ? 3 : 0), gs);
   }
   (void)es;
   return run_oneoff(P, Bd, pr);
@@ -1221,7 +1228,7 @@ static int gemm_batched(DT t, int32_t batch, const char* transa, const int32_t* 
 int hodlr_dgemm_batched(int32_t batch, const char* transa, const int32_t* m, const int32_t* n, const int32_t* k,
                         const double* alpha, const double* const* A, const int64_t* lda, const double* const* B,
                         const int64_t* ldb, const double* beta, double* const* C, const int64_t* ldc, void* stream) {
-  return gemm_batched(F64, batch, transa, m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B, ldb, beta,
+  return gemm_batched(F64, F64, batch, transa, m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B, ldb, beta,
                       (void* const*)C, ldc, stream);
 }
 
@@ -1230,8 +1237,18 @@ int hodlr_sgemm_batched(int32_t batch, const int32_t* m, const int32_t* n, const
                         const double* beta, float* const* C, const int64_t* ldc, void* stream) {
   if (batch <= 0) return HODLR_ERR_ARG;
   std::vector<char> tn(batch, 'n');
-  return gemm_batched(F32, batch, tn.data(), m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B, ldb,
-                      beta, (void* const*)C, ldc, stream);
+  return gemm_batched(F32, F32, batch, tn.data(), m, n, k, alpha, (const void* const*)A, lda, (const void* const*)B,
+                      ldb, beta, (void* const*)C, ldc, stream);
+}
+
+int hodlr_sgemm_batched_bf16(int32_t batch, const int32_t* m, const int32_t* n, const int32_t* k,
+                             const double* alpha, const float* const* A, const int64_t* lda,
+                             const float* const* B, const int64_t* ldb, const double* beta, float* const* C,
+                             const int64_t* ldc, void* stream) {
+  if (batch <= 0) return HODLR_ERR_ARG;
+  std::vector<char> tn(batch, 'n');
+  return gemm_batched(F32, BF16, batch, tn.data(), m, n, k, alpha, (const void* const*)A, lda,
+                      (const void* const*)B, ldb, beta, (void* const*)C, ldc, stream);
 }
 
 int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,

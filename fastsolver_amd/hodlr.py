@@ -18,6 +18,7 @@ from ._lib import PlanInfo, SolveReport, System, check, device_lib
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_BF16 = 2
 PROFILE_NCAT = 20   # HODLR_PROFILE_NCAT
 K_PACKED = 0
 K_STACKED = 1
@@ -363,9 +364,10 @@ def _arr(ctype, vals):
     return (ctype * len(vals))(*vals)
 
 
-def gemm_batched(ops, dtype=np.float64, stream=None):
+def gemm_batched(ops, dtype=np.float64, stream=None, bf16=False):
     """ops: list of (transa, alpha, A, B, beta, C) with DeviceArray operands;
-    C = alpha op(A) B + beta C with op(A) = A ('n') or A^T ('t')."""
+    C = alpha op(A) B + beta C with op(A) = A ('n') or A^T ('t').  float32 with
+    bf16=True runs the products on BF16 MFMA (FP32 accumulation)."""
     lib = device_lib()
     n = len(ops)
     tr = [o[0].encode() for o in ops]
@@ -380,6 +382,8 @@ def gemm_batched(ops, dtype=np.float64, stream=None):
             stream.handle if stream else None)
     if dtype == np.float64:
         check(lib.hodlr_dgemm_batched(n, ctypes.c_char_p(b"".join(tr)), *args), "hodlr_dgemm_batched")
+    elif bf16:
+        check(lib.hodlr_sgemm_batched_bf16(n, *args), "hodlr_sgemm_batched_bf16")
     else:
         check(lib.hodlr_sgemm_batched(n, *args), "hodlr_sgemm_batched")
 

@@ -33,6 +33,9 @@ extern "C" {
 #define HODLR_PREC_FP64 0 /* every phase in FP64 */
 #define HODLR_PREC_FP32 1 /* leaf LU, TRSM, reduce and broadcast in FP32 (MFMA f32), node LU and
                              refinement in FP64 */
+#define HODLR_PREC_BF16 2 /* as FP32, with the leaf LU trailing updates, the TRSM products with
+                             solved rows, the reduce and the broadcast on BF16 MFMA (operands
+                             rounded to BF16, FP32 accumulation and storage) */
 
 #define HODLR_OK 0
 #define HODLR_ERR_ARG -1         /* invalid argument */
@@ -155,6 +158,13 @@ int hodlr_dgemm_batched(int32_t batch, const char* transa, const int32_t* m, con
 int hodlr_sgemm_batched(int32_t batch, const int32_t* m, const int32_t* n, const int32_t* k, const double* alpha,
                         const float* const* A, const int64_t* lda, const float* const* B, const int64_t* ldb,
                         const double* beta, float* const* C, const int64_t* ldc, void* stream);
+/* hodlr_sgemm_batched with the products on BF16 MFMA (v_mfma_f32_16x16x32_bf16):
+ * A and B rounded to BF16, FP32 accumulation and output.  No counterpart in the
+ * reference: the BF16 precision of the factorization (HODLR_PREC_BF16). */
+int hodlr_sgemm_batched_bf16(int32_t batch, const int32_t* m, const int32_t* n, const int32_t* k,
+                             const double* alpha, const float* const* A, const int64_t* lda,
+                             const float* const* B, const int64_t* ldb, const double* beta, float* const* C,
+                             const int64_t* ldc, void* stream);
 int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
                         double* const* B, const int64_t* ldb, int32_t* info, void* stream);
 

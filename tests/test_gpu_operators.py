@@ -78,3 +78,36 @@ def test_integer_known_answers_fp64_and_fp32():
     hodlr.gemm_batched([("n", -1.0, hodlr.DeviceArray.from_host(f(A)), hodlr.DeviceArray.from_host(f(eta)), 1.0, sC)],
                        dtype=np.float32)
     assert np.array_equal(sC.download().astype(np.float64), D)
+
+
+def _round_bf16(x):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(np.shape(x))
+
+
+def test_bf16_gemm_integer_known_answer_and_rounding():
+    """BF16 MFMA products: exact on the integer circulant fixture (entries < 256
+    are exact in BF16), and on random inputs equal to the product of the
+    BF16-rounded operands up to FP32 accumulation."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "circulant_int_gemm.npz"))
+    f = lambda x: np.asfortranarray(np.asarray(x, dtype=np.float32))
+    A, eta, C, D = (f(d[k]) for k in ("A", "eta", "C", "D"))
+    dC = hodlr.DeviceArray.from_host(C)
+    hodlr.gemm_batched([("n", -1.0, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(eta), 1.0, dC)],
+                       dtype=np.float32, bf16=True)
+    assert np.array_equal(dC.download(), D)
+    rng = np.random.default_rng(5)
+    for (m, n, k) in [(300, 100, 200), (64, 32, 64), (1000, 77, 129)]:
+        A = f(rng.standard_normal((m, k)))
+        B = f(rng.standard_normal((k, n)))
+        dC = hodlr.DeviceArray(m, n, np.float32)
+        hodlr.gemm_batched([("n", 1.0, hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(B), 0.0, dC)],
+                           dtype=np.float32, bf16=True)
+        ref = _round_bf16(A).astype(np.float64) @ _round_bf16(B).astype(np.float64)
+        scale = np.abs(_round_bf16(A)).astype(np.float64) @ np.abs(_round_bf16(B)).astype(np.float64)
+        assert np.all(np.abs(dC.download() - ref) <= 1e-6 * scale + 1e-30), (m, n, k)
+        # and it is really BF16: the unrounded product differs by far more
+        assert np.abs(dC.download() - A.astype(np.float64) @ B.astype(np.float64)).max() > 1e-4

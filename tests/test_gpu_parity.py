@@ -143,6 +143,22 @@ def test_fp32_factor_with_fp64_refinement():
     assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
 
 
+@pytest.mark.parametrize("cfg,scale", [(1, 4), (3, 32)])
+def test_bf16_factor_with_fp64_refinement(cfg, scale):
+    """BF16 MFMA products in the leaf LU, TRSM, reduce and broadcast, FP64
+    refinement to the same 1e-12 relative residual."""
+    spec = gen.config(cfg, scale=scale)[0]
+    X, V, K = host_system(spec)
+    Xo = oracle_solve(spec, X, V, K)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_BF16)
+    rep = plan.solve(tol=TOL, maxiter=30)
+    print(f"  bf16: first relres {rep.relres_first:.3e}, {rep.iterations} sweeps, relres {rep.relres:.3e}", flush=True)
+    assert rep.converged and rep.relres <= TOL
+    assert 1e-6 < rep.relres_first < 1e-1      # really a low-precision first solve
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
+
+
 def test_uneven_batch_one_plan():
     """Independent systems of different N, per-level ranks and nrhs in one plan."""
     specs = gen.batched_uneven(6, nmin_blocks=2, nmax_blocks=12)
@@ -208,7 +224,7 @@ def test_autotune_then_solve_matches():
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
-@pytest.mark.parametrize("prec", [hodlr.PREC_FP64, hodlr.PREC_FP32])
+@pytest.mark.parametrize("prec", [hodlr.PREC_FP64, hodlr.PREC_FP32, hodlr.PREC_BF16])
 def test_every_kernel_variant(cfg, prec):
     """Both GEMM tilings and both TRSM variants, odd sizes included."""
     spec = gen.SystemSpec(6000, 400, 5, [70, 40, 33, 20], d_scale=1 / 16)
@@ -217,6 +233,6 @@ def test_every_kernel_variant(cfg, prec):
     ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
     plan = hodlr.Plan([ds], prec)
     plan.force_tiling(cfg)
-    rep = plan.solve(tol=TOL, maxiter=10)
+    rep = plan.solve(tol=TOL, maxiter=30)
     assert rep.converged
     assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= (TOL if prec == hodlr.PREC_FP64 else 1e-11)
