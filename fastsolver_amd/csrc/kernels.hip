@@ -491,29 +491,38 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
         gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
       else
         gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
-      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) {
+      // right-hand-side rows of the step, read coalesced (64 rows of a column
+      // per wave) into s_t; then s_t -= the product with the solved rows
+      for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
+        const int r = e % RB, c = e / RB;
         MT x = 0;
         if (r < nr && c < nc) {
           if (pass == 0) {
             const int src = perm ? perm[r0 + r] : r0 + r;
-            x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
+            x = MT(Xin[src + int64_t(c0 + c) * ldin]);
           } else {
-            x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]) - v;
+            x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]);
           }
         }
         s_t[r * P + c] = x;
-      });
+      }
       const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
       for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
         const int i = e % RB, k = e / RB;
         s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
       }
       __syncthreads();
+      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) { s_t[r * P + c] -= MT(v); });
+      __syncthreads();
       t.zero();
       t.template mma<RB>(s_u, s_t, P, P);
-      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(v);
-      });
+      __syncthreads();   // every wave has read s_t
+      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * P + c] = v; });
+      __syncthreads();
+      for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
+        const int r = e % RB, c = e / RB;
+        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * P + c]);
+      }
       __syncthreads();
     }
   }
