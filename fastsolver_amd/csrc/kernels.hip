@@ -568,16 +568,19 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
     for (int e = threadIdx.x; e < HB * HB; e += kThreads) s_u[(e / HB) * PI + e % HB] = inv[e];
   };
   // rows [h*64, h*64+64) of s_t <- inv * rows of s_t; also written to Xout
+  // (coalesced: 64 rows of a column per wave)
   auto finish_half = [&](int h, int R0, int nr_h) {
     Half t;
     t.zero();
     t.template mma<HB>(s_u, s_t + h * HB * PT, PI, PT);
     __syncthreads();   // every wave has read the half before it is overwritten
-    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
-      s_t[(h * HB + r) * PT + c] = v;
-      if (r < nr_h && c < nc) Xout[(R0 + h * HB + r) + int64_t(c0 + c) * ldout] = TOUT(v);
-    });
+    t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[(h * HB + r) * PT + c] = v; });
     __syncthreads();
+    for (int e = threadIdx.x; e < HB * CB; e += kThreads) {
+      const int r = e % HB, c = e / HB;
+      if (r < nr_h && c < nc) Xout[(R0 + h * HB + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[(h * HB + r) * PT + c]);
+    }
+    __syncthreads();   // s_t is read before the next step refills it
   };
   // rows of half `dst` of s_t -= block(s_u) * rows of half `src`
   auto couple = [&](int dst, int src) {
@@ -600,18 +603,22 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
           gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, R0, m, Xout, ldout, c0, ncols, 0, R0);
         else
           gemm_loop<MT, RB, CB, BK, false>(t, s_u, LU, ldlu, R0, R0 + nr, Xout, ldout, c0, ncols, R0 + RB, m);
-        t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) {
+        // right-hand-side rows read coalesced into s_t, then s_t -= product
+        for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
+          const int r = e % RB, c = e / RB;
           MT x = 0;
           if (r < nr && c < nc) {
             if (pass == 0) {
               const int src = perm ? perm[R0 + r] : R0 + r;
-              x = MT(Xin[src + int64_t(c0 + c) * ldin]) - v;
+              x = MT(Xin[src + int64_t(c0 + c) * ldin]);
             } else {
-              x = MT(Xout[(R0 + r) + int64_t(c0 + c) * ldout]) - v;
+              x = MT(Xout[(R0 + r) + int64_t(c0 + c) * ldout]);
             }
           }
           s_t[r * PT + c] = x;
-        });
+        }
+        __syncthreads();
+        t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * PT + c] -= v; });
       }
       const int b0 = 2 * st;   // 64-row diagonal block of the first half
       if (pass == 0) {
