@@ -15,7 +15,9 @@ One step = one complete solve of the system for all its right-hand sides:
     residuals and FP32 correction solves until max_j ||b_j - A x_j|| / ||b_j||
     <= 1e-12, host checks included;
   * fp64 (reported beside it): one fused FP64 factor+solve pass, the
-    reference's bfs_solve; its residual is checked after the timed region.
+    reference's bfs_solve; its residual is checked after the timed region;
+  * mixed_bf16 (reported beside it): as mixed, with the leaf updates, TRSM
+    updates, reduce and broadcast on BF16 MFMA (more refinement sweeps).
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -139,6 +141,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--maxiter", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1)
@@ -223,6 +226,32 @@ def main():
     xmix = {i: dss[i].solution() for i in sample}
     log(f"[bench] mixed: {tmix * 1e3:.2f} ms/solve, {rep.iterations} refinement sweeps, "
         f"relres {rep.relres:.2e} (first {rep.relres_first:.2e}); dominant kernel {dom}")
+    p32.close()
+    del p32
+
+    # ------------------------------------------------------------ BF16 + refine (reported beside)
+    bf = None
+    xbf = {}
+    if not args.no_bf16:
+        pbf = hodlr.Plan(dss, hodlr.PREC_BF16)
+        if args.tune:
+            pbf.autotune(args.tune)
+        if args.graph:
+            pbf.use_graph(True)
+        bf_box = {}
+
+        def mixed_bf16():
+            bf_box["r"] = pbf.solve(TOL, args.maxiter)
+            return bf_box["r"]
+
+        tbf, rbf = timed_steps(mixed_bf16, args.steps, args.warmup)
+        xbf = {i: dss[i].solution() for i in sample}
+        bf = {"ms": round(tbf * 1e3, 3), "solves_per_s": round(1.0 / tbf, 3), "iterations": rbf.iterations,
+              "relres": rbf.relres, "relres_first": rbf.relres_first}
+        log(f"[bench] bf16: {tbf * 1e3:.2f} ms/solve, {rbf.iterations} refinement sweeps, relres {rbf.relres:.2e} "
+            f"(first {rbf.relres_first:.2e})")
+        pbf.close()
+        del pbf
     roof = roofline_entry(prof_dom, dom, "fp32" if dom not in ("node_gemm", "node_lu", "node_trsm", "matvec",
                                                                 "reduce_sum", "norm", "solve_node") else "fp64")
     if roof is not None:
@@ -237,6 +266,7 @@ def main():
                   "relres_first": rep.relres_first},
         "fp64": {"ms": round(t64 * 1e3, 3), "solves_per_s": round(1.0 / t64, 3), "relres": relres64,
                  "gflops": round(info64.flops_factor / t64 / 1e9, 1)},
+        "mixed_bf16": bf,
         "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
         "flops_factor": info64.flops_factor,
         "launches_fp64_pass": info64.launches_factor,
@@ -246,25 +276,28 @@ def main():
         "profile_mixed_ms": {k: round(v["ms"], 3) for k, v in prof_all.items() if v["launches"]},
         "cpu_baseline": None,
     }
-    p32.close()
-    del p32, dss
+    del dss
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline: oracle on {threads} host threads, {len(sample)} of {len(specs)} systems ...")
-        tcpu, e64, emix, rcs, lapack = 0.0, 0.0, 0.0, [], ""
+        tcpu, e64, emix, ebf, rcs, lapack = 0.0, 0.0, 0.0, 0.0, [], ""
         for i in sorted(sample):
             rc, t, xref, lapack = cpu_baseline(specs[i], *hosts[i], threads)
             tcpu += t
             rcs.append(rc)
             e64 = max(e64, float(np.linalg.norm(x64[i] - xref) / np.linalg.norm(xref)))
             emix = max(emix, float(np.linalg.norm(xmix[i] - xref) / np.linalg.norm(xref)))
+            if xbf:
+                ebf = max(ebf, float(np.linalg.norm(xbf[i] - xref) / np.linalg.norm(xref)))
         # a sampled batch is scaled to the whole batch by the algorithmic flops
         fl = lambda sp: hodlr_flops(sp)
         scale = sum(fl(sp) for sp in specs) / sum(fl(specs[i]) for i in sample)
         tfull = tcpu * scale
         out["fp64"]["err_vs_oracle"] = e64
         out["mixed"]["err_vs_oracle"] = emix
+        if bf is not None:
+            out["mixed_bf16"]["err_vs_oracle"] = ebf
         what = (f"one full solve of the same workload (status {max(rcs)})" if not batch else
                 f"full solves of {len(sample)} of the {len(specs)} systems ({tcpu:.2f} s, status {max(rcs)}), "
                 f"scaled to the batch by algorithmic flops (x{scale:.2f})")

@@ -25,7 +25,8 @@ namespace hodlr {
 template <int ROLE, class TA, class TB, class TC, class MT, bool TRANS, int BM, int BN>
 __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
   constexpr int BK = sizeof(MT) == 2 ? kGemmBKbf16
-                                    : (BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32));
+                   : BN == kGemmNarrowBN ? kGemmNarrowBK
+                   : (BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32));
   typedef Tile<MT, BM, BN, BK> TileT;
   typedef typename Mfma<MT>::acc_e AE;
   // the epilogue stages CH columns of the accumulator tile through the same LDS

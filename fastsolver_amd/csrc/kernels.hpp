@@ -14,6 +14,7 @@ constexpr int kGemmBigBK = 16;      // k step of the 128 x 128 tiles and of the 
 constexpr int kGemmBKbf16 = 32;     // k step of bf16 tiles (one 16x16x32 MFMA)
 constexpr int kTrsmBK = 16, kTrsmBK32 = 16;   // k step of the 64-row TRSM (fp64, fp32)
 constexpr int kGemmNarrowBN = 32;   // tile 2: 64 x 32 for problems at most 32 columns wide
+constexpr int kGemmNarrowBK = 32;   // k step of the narrow tiles (HBM-bound streaming products)
 constexpr int kTrsmRB = 64, kTrsmCB = 64;
 constexpr int kLuNB = 64;          // outer panel width of the blocked LU
 constexpr int kCopyTile = 1024;    // elements per copy workgroup
