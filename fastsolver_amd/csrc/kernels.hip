@@ -710,6 +710,10 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
     if (role == 2) HODLR_GEMM3(2, float, float, float, float, false);
     else HODLR_GEMM3(3, float, float, float, float, false);
+  } else if (ta == F32 && tb == F32 && tc == F64 && mt == F32 && trans) {
+    HODLR_GEMM3(1, float, float, double, float, true);
+  } else if (ta == F32 && tb == F32 && tc == F64 && mt == BF16 && trans) {
+    HODLR_GEMM3(1, float, float, double, bf16, true);
   } else if (ta == F64 && tb == F32 && tc == F64 && mt == BF16 && trans) {
     HODLR_GEMM3(1, double, float, double, bf16, true);
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == BF16 && !trans) {

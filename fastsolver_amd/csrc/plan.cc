@@ -88,6 +88,7 @@ struct SysState {
   std::vector<int32_t> ranks;
   int64_t ldw = 0;                 // leading dimension of the workspaces (= N)
   size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0, ldinv = 0;
+  size_t vws = 0;                  // FP32 plans: V rounded to FP32 once per factor pass (N x R, ld N)
   std::vector<size_t> lu_off;      // per leaf (elements)
   std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
   std::vector<size_t> z_off;       // per leaf (doubles)
@@ -430,6 +431,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     S.xws = L.take(size_t(n) * t.xcols() * we);
     S.yws = L.take(size_t(n) * c * we);
     S.rbuf = L.take(size_t(n) * c * 8);
+    if (we != 8) S.vws = L.take(size_t(n) * std::max<int64_t>(t.vcols(), 1) * we);
     S.lpiv = L.take(size_t(n) * 4);
     S.lperm = L.take(size_t(n) * 4);
     size_t lu_elems = 0, z_elems = 0, k_elems = 0;
@@ -538,6 +540,17 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       }
       leaf_base += int(t.leaves.size());
     }
+    if (wt != F64) {
+      // V is read by every column tile of the reduce: round it to FP32 once
+      std::vector<CopyDesc> vc;
+      for (SysState& S : P->sys) {
+        CopyDesc c{};
+        c.src = S.s.V; c.lds = S.s.ldv; c.dst = P->ws.p + S.vws; c.ldd = S.ldw;
+        c.rows = int(S.tree.n); c.cols = int(S.tree.vcols()); c.alpha = 1.0;
+        vc.push_back(c);
+      }
+      B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, vc);
+    }
     B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, kc);
     B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus, P->mt);
     B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr, P->mt);
@@ -571,7 +584,11 @@ void build_all(hodlr_plan_s* P, Builder& B) {
           for (int k = 0; k < 2; ++k) {
             const TreeNode& C = t.nodes[ch[k]];
             Builder::G g{};
-            g.A = S.s.V + C.row_beg + vcol * S.s.ldv; g.lda = S.s.ldv;
+            if (wt == F64) {
+              g.A = S.s.V + C.row_beg + vcol * S.s.ldv; g.lda = S.s.ldv;
+            } else {
+              g.A = P->ws.p + S.vws + (size_t(C.row_beg) + size_t(vcol) * S.ldw) * we; g.lda = S.ldw;
+            }
             g.B = X + size_t(C.row_beg) * we; g.ldb = S.ldw;
             g.Cout = vtd[k]; g.ldcout = r; g.Cin = nullptr; g.ldcin = r;
             g.m = r; g.n = nd; g.k = int(C.nrow); g.alpha = 1.0; g.beta = 0.0;
@@ -646,7 +663,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       const Cat cg = mode == 0 ? CAT_NODE_GEMM : CAT_SOLVE_NODE;
       const Cat ct = mode == 0 ? CAT_NODE_TRSM : CAT_SOLVE_NODE;
       const Cat cb = mode == 0 ? CAT_BCAST : CAT_SOLVE_BCAST;
-      B.gemm(pr, cr, F64, wt, F64, P->mt, true, 1, red, 2048);
+      B.gemm(pr, cr, wt, wt, F64, P->mt, true, 1, red, 2048);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng1);
       if (mode == 0) {
         B.copy(pn, CAT_NODE_LU, F64, F64, sform);
