@@ -458,6 +458,7 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
   // type of the products with the solved rows (MT, or bf16 in the bf16 plans)
   typedef TLU MT;
   constexpr int BK = sizeof(MT) == 8 ? kTrsmBK : kTrsmBK32, RB = kTrsmRB, CB = kTrsmCB, P = RB + 1;
+  constexpr int PT = CB + 1;   // pitch of the right-hand-side block
   constexpr int BKU = sizeof(UT) == 2 ? kGemmBKbf16 : BK;
   static_assert(RB == kDB, "diagonal blocks and TRSM steps share the 64-row blocking");
   typedef Tile<MT, RB, CB, BK> TileT;
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
   // MFMA stages of the update, then the inverted diagonal block (As layout)
   __shared__ __attribute__((aligned(16))) char s_u_raw[UBYTES > DBYTES ? UBYTES : DBYTES];
   MT* s_u = (MT*)s_u_raw;
-  __shared__ MT s_t[RB * P];  // right-hand-side block, s_t[r*P + c] (Bs layout)
+  __shared__ MT s_t[RB * PT];  // right-hand-side block, s_t[r*PT + c] (Bs layout)
 
   const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const TrsmDesc d = descs[find_desc(descs, ndesc, bid)];
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
             x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]);
           }
         }
-        s_t[r * P + c] = x;
+        s_t[r * PT + c] = x;
       }
       const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
       for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
@@ -513,16 +514,16 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
         s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
       }
       __syncthreads();
-      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) { s_t[r * P + c] -= MT(v); });
+      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) { s_t[r * PT + c] -= MT(v); });
       __syncthreads();
       t.zero();
-      t.template mma<RB>(s_u, s_t, P, P);
+      t.template mma<RB>(s_u, s_t, P, PT);
       __syncthreads();   // every wave has read s_t
-      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * P + c] = v; });
+      t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * PT + c] = v; });
       __syncthreads();
       for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
         const int r = e % RB, c = e / RB;
-        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * P + c]);
+        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * PT + c]);
       }
       __syncthreads();
     }
