@@ -93,3 +93,20 @@ def test_cxx_facade_compiles_and_example_links():
     ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
     assert "libfastsolver_amd.so" in ldd and "libfastsolver_amd_gen.so" in ldd
     assert "not found" not in ldd, ldd
+
+
+def _sys(n, threshold, nrhs, ranks):
+    r = np.array(ranks, np.int32)
+    # dummy non-null device pointers: validation fails before any is touched
+    return _lib.System(n, threshold, nrhs, len(r), r.ctypes.data, 8, n, 8, n, 8, 0, 0, 0, 8, n), r
+
+
+def test_plan_create_rejects_unsupported_trees_before_touching_the_gpu():
+    lib = _lib.device_lib()
+    h = ctypes.c_void_p()
+    s, keep = _sys(4096, 2048, 1, [30])          # leaves of 2048 rows: above the 1024 the kernels hold
+    assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -3
+    s, keep = _sys(256, 64, 1, [64])             # leaf rows <= rank (hodlr_matrix.cc:167 asserts N > rank)
+    assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -2
+    s, keep = _sys(1 << 12, 256, 2, [20, 20])    # per-depth ranks must cover every level
+    assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -2
