@@ -217,6 +217,35 @@ struct Stager {
       ra[s] = *(ok ? A + off : zero_page<TA>());
     }
   }
+  // Interior tiles (every element in range): a wave-uniform base pointer plus
+  // a per-thread 32-bit offset that does not change with the k step, so each
+  // load needs no address arithmetic and no range check.  Element e = tid +
+  // s*kThreads of the tile has inner index e % INNER = tid % INNER and outer
+  // index tid / INNER + s * (kThreads / INNER).
+  __device__ __forceinline__ void load_a_full(const HODLR_GLOBAL TA* A, int64_t lda, int r0, int k0) {
+    constexpr int INNER = TRANS ? BK : BM, STEP = kThreads / INNER;
+    static_assert(kThreads % INNER == 0, "tile shape");
+    const int tid = threadIdx.x;
+    const uint32_t voff = uint32_t(tid % INNER) + uint32_t(tid / INNER) * uint32_t(lda);
+    const HODLR_GLOBAL TA* base = TRANS ? A + k0 + int64_t(r0) * lda : A + r0 + int64_t(k0) * lda;
+#pragma unroll
+    for (int s = 0; s < NA; ++s) ra[s] = (base + int64_t(s) * STEP * lda)[voff];
+  }
+  __device__ __forceinline__ void load_b_full(const HODLR_GLOBAL TB* B, int64_t ldb, int k0, int c0) {
+    constexpr int INNER = BK, STEP = kThreads / INNER;
+    const int tid = threadIdx.x;
+    const uint32_t voff = uint32_t(tid % INNER) + uint32_t(tid / INNER) * uint32_t(ldb);
+    const HODLR_GLOBAL TB* base = B + k0 + int64_t(c0) * ldb;
+#pragma unroll
+    for (int s = 0; s < NB; ++s) rb[s] = (base + int64_t(s) * STEP * ldb)[voff];
+  }
+  __device__ __forceinline__ void load(const HODLR_GLOBAL TA* A, int64_t lda, int r0, int mlim,
+                                       const HODLR_GLOBAL TB* B, int64_t ldb, int c0, int nlim, int k0, int klim) {
+    if (r0 + BM <= mlim && k0 + BK <= klim) load_a_full(A, lda, r0, k0);
+    else load_a(A, lda, r0, mlim, k0, klim);
+    if (c0 + BN <= nlim && k0 + BK <= klim) load_b_full(B, ldb, k0, c0);
+    else load_b(B, ldb, k0, klim, c0, nlim);
+  }
   __device__ __forceinline__ void load_b(const HODLR_GLOBAL TB* B, int64_t ldb, int k0, int klim, int c0, int nlim) {
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
@@ -273,19 +302,39 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
   typedef Tile<MT, BM, BN, BK> T;
   if (kb >= ke) return;
   Stager<MT, TA, TB, BM, BN, BK, TRANS> st;
-  st.load_a(A, lda, r0, mlim, kb, ke);
-  st.load_b(B, ldb, kb, ke, c0, nlim);
+  st.load(A, lda, r0, mlim, B, ldb, c0, nlim, kb, ke);
   st.store(lds, lds + T::BOFF);
   __syncthreads();
   int stage = 0;
-  for (int k0 = kb; k0 < ke; k0 += BK) {
+  int k0 = kb;
+  // steps whose successor is a full k step of an interior tile: branch-free loads
+  if (r0 + BM <= mlim && c0 + BN <= nlim) {
+    for (; k0 + 2 * BK <= ke; k0 += BK) {
+      st.load_a_full(A, lda, r0, k0 + BK);
+      st.load_b_full(B, ldb, k0 + BK, c0);
+      // the compiler otherwise sinks the loads down to their LDS store, after
+      // the MFMAs, and waits for them there: keep them issued ahead of the
+      // product and the store (with its wait) behind it
+      __builtin_amdgcn_sched_barrier(0);
+      MT* cur = lds + stage * T::STAGE;
+      t.template mma<BK>(cur, cur + T::BOFF);
+      __builtin_amdgcn_sched_barrier(0);
+      MT* nxt = lds + (stage ^ 1) * T::STAGE;
+      st.store(nxt, nxt + T::BOFF);
+      __syncthreads();
+      stage ^= 1;
+    }
+  }
+  for (; k0 < ke; k0 += BK) {
     const bool more = k0 + BK < ke;
     if (more) {
       st.load_a(A, lda, r0, mlim, k0 + BK, ke);
       st.load_b(B, ldb, k0 + BK, ke, c0, nlim);
     }
+    __builtin_amdgcn_sched_barrier(0);
     MT* cur = lds + stage * T::STAGE;
     t.template mma<BK>(cur, cur + T::BOFF);
+    __builtin_amdgcn_sched_barrier(0);
     if (more) {
       MT* nxt = lds + (stage ^ 1) * T::STAGE;
       st.store(nxt, nxt + T::BOFF);
