@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--maxiter", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--tune", type=int, default=2, help="autotuner rounds before timing (0 = off)")
+    ap.add_argument("--tiling", type=int, default=-1,
+                    help="with --tune 0: force every tunable launch to its smaller (0) or larger (1) choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported "
@@ -178,6 +180,8 @@ def main():
     # ------------------------------------------------------------ plain FP64
     p64 = hodlr.Plan(dss, hodlr.PREC_FP64)
     info64 = p64.info()
+    if not args.tune and args.tiling >= 0:
+        p64.force_tiling(args.tiling)
     if args.tune:
         big = p64.autotune(args.tune)
         log(f"[bench] fp64 autotune: {big} launches on 128x128 tiles")
@@ -198,6 +202,8 @@ def main():
     # ------------------------------------------------------------ mixed + refine
     p32 = hodlr.Plan(dss, hodlr.PREC_FP32)
     ws32 = p32.info().workspace_bytes
+    if not args.tune and args.tiling >= 0:
+        p32.force_tiling(args.tiling)
     if args.tune:
         big = p32.autotune(args.tune)
         log(f"[bench] mixed autotune: {big} launches on 128x128 tiles")
@@ -236,6 +242,8 @@ def main():
         pbf = hodlr.Plan(dss, hodlr.PREC_BF16)
         if args.tune:
             pbf.autotune(args.tune)
+        elif args.tiling >= 0:
+            pbf.force_tiling(args.tiling)
         if args.graph:
             pbf.use_graph(True)
         bf_box = {}
