@@ -110,3 +110,11 @@ def test_plan_create_rejects_unsupported_trees_before_touching_the_gpu():
     assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -2
     s, keep = _sys(1 << 12, 256, 2, [20, 20])    # per-depth ranks must cover every level
     assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -2
+
+
+def test_plan_create_rejects_empty_and_degenerate_systems():
+    lib = _lib.device_lib()
+    h = ctypes.c_void_p()
+    for (n, thr, c) in [(0, 64, 1), (256, 0, 1), (256, 64, 0)]:
+        s, keep = _sys(n, thr, c, [8])
+        assert lib.hodlr_plan_create(ctypes.byref(h), ctypes.byref(s), 1, 0, None) == -1, (n, thr, c)
