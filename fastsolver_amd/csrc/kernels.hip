@@ -158,6 +158,45 @@ __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
   if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
 }
 
+// (value, index) argmax over the 64 lanes of a wave with DPP row shifts and
+// row broadcasts (VALU latency) instead of LDS-routed shuffles: after
+// row_shr 1/2/4/8 lane 15 of each row holds its row's best, row_bcast 15/31
+// carry it to lane 63, which every lane then reads.  Ties keep the smaller
+// index, as pivot_combine does.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i32(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, ROWS, 0xf, false); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_val(float x) { return __int_as_float(dpp_i32<CTRL, ROWS>(__float_as_int(x))); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_val(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i32<CTRL, ROWS>(int(b)), hi = dpp_i32<CTRL, ROWS>(int(b >> 32));
+  return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+template <class T, int CTRL, int ROWS>
+__device__ __forceinline__ void argmax_step(T& bv, int& bi) {
+  const T ov = dpp_val<CTRL, ROWS>(bv);
+  const int oi = dpp_i32<CTRL, ROWS>(bi);
+  pivot_combine(bv, bi, ov, oi);
+}
+template <class T>
+__device__ __forceinline__ void wave_argmax(T& bv, int& bi) {
+  argmax_step<T, 0x111, 0xf>(bv, bi);   // row_shr:1
+  argmax_step<T, 0x112, 0xf>(bv, bi);   // row_shr:2
+  argmax_step<T, 0x114, 0xf>(bv, bi);   // row_shr:4
+  argmax_step<T, 0x118, 0xf>(bv, bi);   // row_shr:8
+  argmax_step<T, 0x142, 0xa>(bv, bi);   // row_bcast:15 into rows 1 and 3
+  argmax_step<T, 0x143, 0xc>(bv, bi);   // row_bcast:31 into rows 2 and 3
+  if constexpr (sizeof(T) == 8) {
+    const long long b = __double_as_longlong(bv);
+    const int lo = __builtin_amdgcn_readlane(int(b), 63), hi = __builtin_amdgcn_readlane(int(b >> 32), 63);
+    bv = __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+  } else {
+    bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bv), 63));
+  }
+  bi = __builtin_amdgcn_readlane(bi, 63);
+}
+
 template <class PT>
 __device__ __forceinline__ void swap_rows_in_column(PT A, int64_t col, int r1, int r2) {
   if (r1 != r2) {
@@ -208,12 +247,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
         T bv = (mine && tid >= j) ? T(fabs(a[j])) : T(-1);
         int bi = (mine && tid >= j) ? tid : 0x7fffffff;
         if (!(bv == bv)) { bv = T(-1); bi = 0x7fffffff; }   // NaN rows never win
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          T ov = __shfl_xor(bv, off);
-          int oi = __shfl_xor(bi, off);
-          pivot_combine(bv, bi, ov, oi);
-        }
+        wave_argmax(bv, bi);
         if (lane == 0) { s_rv[buf][wave] = bv; s_ri[buf][wave] = bi; }
         __syncthreads();
         T pv = s_rv[buf][0];

@@ -111,3 +111,19 @@ def test_bf16_gemm_integer_known_answer_and_rounding():
         assert np.all(np.abs(dC.download() - ref) <= 1e-6 * scale + 1e-30), (m, n, k)
         # and it is really BF16: the unrounded product differs by far more
         assert np.abs(dC.download() - A.astype(np.float64) @ B.astype(np.float64)).max() > 1e-4
+
+
+@pytest.mark.parametrize("n", [64, 200, 1024])
+def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
+    """The in-place LU factors equal LAPACK getrf's: the pivot search picks the
+    same rows (first largest |a|, dgetf2's idamax), so L and U agree to rounding
+    (a different pivot anywhere would change them completely)."""
+    import scipy.linalg
+    rng = np.random.default_rng(100 + n)
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    lu, piv = scipy.linalg.lu_factor(A)
+    dA, dB = hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(np.ones((n, 1)))
+    info = hodlr.dgesv_batched([dA], [dB])
+    assert info[0] == 0
+    got = dA.download()
+    assert np.abs(got - lu).max() <= 1e-10 * np.abs(lu).max()
