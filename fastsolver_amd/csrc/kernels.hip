@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 
 // ========================================================================= LU
 // Panel of kLuNB columns starting at k0, factored as kLuNB/PW sub-panels of PW
-// columns held in registers: NT threads (256, 512 or 1024, the smallest that
+// columns held in registers: NT threads (128, 256, 512 or 1024, the smallest that
 // covers the batch's largest block), thread t owns row ks + t of the sub-panel.  PW = 16 for fp64 and 32 for fp32
 // keeps the register panel within the 128 VGPRs a 1024-thread group allows.
 // Partial pivoting picks the first largest |a| (idamax); the column below a
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restr
 // diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
 // serial substitution.
 template <class TLU, class TIN, class TOUT, class UT>
-__global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(TLU) == 8 ? 3 : 4))) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
   // MT: type of the diagonal-block products and of the stored factor; UT: math
   // type of the products with the solved rows (MT, or bf16 in the bf16 plans)
   typedef TLU MT;
@@ -498,10 +498,17 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
   typedef Tile<MT, RB, CB, BK> TileT;
   typedef Tile<UT, RB, CB, BKU> TileU;
   constexpr int UBYTES = TileU::LDS_ELEMS * int(sizeof(UT)), DBYTES = RB * P * int(sizeof(MT));
-  // MFMA stages of the update, then the inverted diagonal block (As layout)
-  __shared__ __attribute__((aligned(16))) char s_u_raw[UBYTES > DBYTES ? UBYTES : DBYTES];
-  MT* s_u = (MT*)s_u_raw;
-  __shared__ MT s_t[RB * PT];  // right-hand-side block, s_t[r*PT + c] (Bs layout)
+  constexpr int TBYTES = RB * PT * int(sizeof(MT)), HK = RB / 2, HBYTES = HK * P * int(sizeof(MT));
+  constexpr int UDB = UBYTES > DBYTES ? UBYTES : DBYTES;
+  // fp64: the right-hand-side block reuses the MFMA stages (it is filled after
+  // the update loop) and the inverted diagonal block is applied in two k
+  // halves, which keeps a workgroup under 53 KB of LDS (3 per CU instead of 2)
+  constexpr bool COMPACT = sizeof(MT) == 8;
+  constexpr int LBYTES = COMPACT ? (UBYTES > TBYTES + HBYTES ? UBYTES : TBYTES + HBYTES) : UDB + TBYTES;
+  __shared__ __attribute__((aligned(16))) char lds_raw[LBYTES];
+  UT* s_stage = (UT*)lds_raw;                                  // MFMA stages of the update
+  MT* s_t = (MT*)(lds_raw + (COMPACT ? 0 : UDB));            // right-hand-side block, s_t[r*PT + c]
+  MT* s_u = (MT*)(lds_raw + (COMPACT ? TBYTES : 0));         // inverted diagonal block (As layout)
 
   const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const TrsmDesc d = descs[find_desc(descs, ndesc, bid)];
@@ -524,9 +531,9 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
       const int r0 = b * RB, nr = min(RB, m - r0);
       tu.zero();
       if (pass == 0)
-        gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
+        gemm_loop<UT, RB, CB, BKU, false>(tu, s_stage, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
       else
-        gemm_loop<UT, RB, CB, BKU, false>(tu, (UT*)s_u_raw, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
+        gemm_loop<UT, RB, CB, BKU, false>(tu, s_stage, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
       // right-hand-side rows of the step, read coalesced (64 rows of a column
       // per wave) into s_t; then s_t -= the product with the solved rows
       for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
@@ -543,7 +550,8 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
         s_t[r * PT + c] = x;
       }
       const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
-      for (int e = threadIdx.x; e < RB * RB; e += kThreads) {
+      constexpr int KL = COMPACT ? HK : RB;   // k rows of the inverse held at once
+      for (int e = threadIdx.x; e < RB * KL; e += kThreads) {
         const int i = e % RB, k = e / RB;
         s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
       }
@@ -551,8 +559,17 @@ __global__ __launch_bounds__(kThreads, 4) void trsm_kernel(const TrsmDesc* __res
       tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) { s_t[r * PT + c] -= MT(v); });
       __syncthreads();
       t.zero();
-      t.template mma<RB>(s_u, s_t, P, PT);
-      __syncthreads();   // every wave has read s_t
+      t.template mma<KL>(s_u, s_t, P, PT);
+      __syncthreads();   // every wave has read s_u (and s_t)
+      if constexpr (COMPACT) {
+        for (int e = threadIdx.x; e < RB * HK; e += kThreads) {
+          const int i = e % RB, k = e / RB;
+          s_u[k * P + i] = inv[e + RB * HK];   // inverse columns HK .. RB-1
+        }
+        __syncthreads();
+        t.template mma<HK>(s_u, s_t + HK * PT, P, PT);
+        __syncthreads();
+      }
       t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * PT + c] = v; });
       __syncthreads();
       for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
@@ -788,7 +805,8 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
     if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT>), g, dim3(NT), 0, s, d, k0); \
     else hipLaunchKernelGGL((lu_panel_kernel<float, NT>), g, dim3(NT), 0, s, d, k0);         \
   } while (0)
-  if (rows_max <= 256) HODLR_PANEL(256);
+  if (rows_max <= 128) HODLR_PANEL(128);
+  else if (rows_max <= 256) HODLR_PANEL(256);
   else if (rows_max <= 512) HODLR_PANEL(512);
   else HODLR_PANEL(1024);
 #undef HODLR_PANEL
