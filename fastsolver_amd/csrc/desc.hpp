@@ -42,7 +42,7 @@ struct LuDesc {
   int32_t* perm;      // m entries, row i of P*A is row perm[i] of A
   int32_t* info;      // first zero pivot (1-based), 0 if none
   void* dinv;         // per 64-row diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 each (ld 64)
-  int64_t tile_begin; // used by the row-block launch
+  int64_t pad1;       // keeps the descriptor at 64 bytes
 };
 
 // X_out = U^-1 L^-1 P X_in for one LU-factored block and ncols right-hand sides.

@@ -351,45 +351,6 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
   if (tid == 0 && s_info != 0 && *d.info == 0) *d.info = s_info;
 }
 
-// U12 = L11^-1 A12 for the kLuNB rows of the panel at k0, 64 columns per block.
-template <class T>
-__global__ __launch_bounds__(kThreads) void lu_rowblock_kernel(const LuDesc* __restrict__ descs, int k0) {
-  const LuDesc d = descs[blockIdx.y];
-  const int m = d.m;
-  if (k0 >= m) return;
-  const int nb = min(kLuNB, m - k0);
-  const int c0 = k0 + nb + blockIdx.x * 64;
-  if (c0 >= m) return;
-  const int nc = min(64, m - c0);
-  HODLR_GLOBAL T* A = glob((T*)d.A);
-  const int64_t lda = d.lda;
-  constexpr int P = 65;
-  __shared__ T s_l[kLuNB * P];
-  __shared__ T s_x[kLuNB * P];
-  for (int e = threadIdx.x; e < nb * nb; e += kThreads) {
-    const int r = e % nb, k = e / nb;
-    s_l[r * P + k] = A[(k0 + r) + int64_t(k0 + k) * lda];
-  }
-  for (int e = threadIdx.x; e < nb * nc; e += kThreads) {
-    const int r = e % nb, c = e / nb;
-    s_x[r * P + c] = A[(k0 + r) + int64_t(c0 + c) * lda];
-  }
-  __syncthreads();
-  if (threadIdx.x < nc) {
-    const int c = threadIdx.x;
-    for (int i = 1; i < nb; ++i) {
-      T x = s_x[i * P + c];
-      for (int k = 0; k < i; ++k) x -= s_l[i * P + k] * s_x[k * P + c];
-      s_x[i * P + c] = x;
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < nb * nc; e += kThreads) {
-    const int r = e % nb, c = e / nb;
-    A[(k0 + r) + int64_t(c0 + c) * lda] = s_x[r * P + c];
-  }
-}
-
 // perm from the sequential interchanges ipiv (row i of P*A is row perm[i] of A)
 constexpr int kPermMax = 8192;
 __global__ __launch_bounds__(64) void lu_perm_kernel(const LuDesc* __restrict__ descs) {
@@ -428,56 +389,64 @@ __global__ void lu_init_kernel(const LuDesc* __restrict__ descs, int n) {
 }
 
 // ====================================================== diagonal-block inverses
-// For every 64-row diagonal block b of an LU factor: L_bb^-1 (unit lower) and
-// U_bb^-1 (upper), zero outside the block's nr x nr corner.  Threads 0..63
-// invert L by columns, threads 64..127 invert U; each column is one forward /
-// backward substitution against a unit vector.
+// For the 64-row diagonal block b0 + blockIdx.x of every LU factor: L_bb^-1
+// (unit lower) and U_bb^-1 (upper), zero outside the block's nr x nr corner.
+// The block is final once its panel is factored, so the plan inverts it right
+// after the panel: the row-block solve U12 = L11^-1 A12 and the leaf TRSM
+// then run as MFMA products.  Wave 0 builds column j = lane of L^-1, wave 1
+// column j of U^-1, each column held in registers and updated column-oriented
+// (x_i -= A_ik x_k for all i at once), so the 64-step chains carry up to 63
+// independent FMAs per step and the block entries are wave-uniform LDS reads.
 constexpr int kDB = 64;
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void lu_dinv_kernel(const LuDesc* __restrict__ descs) {
+__global__ __launch_bounds__(128) void lu_dinv_kernel(const LuDesc* __restrict__ descs, int b0) {
   const LuDesc d = descs[blockIdx.y];
-  const int b = blockIdx.x;
+  const int b = b0 + blockIdx.x;
   const int r0 = b * kDB;
   if (r0 >= d.m) return;
   const int nr = min(kDB, d.m - r0);
   const HODLR_GLOBAL T* A = glob((const T*)d.A);
-  constexpr int P = kDB + 1;
-  __shared__ T s_a[kDB * P];   // the diagonal block, s_a[i*P + k] = A(r0+i, r0+k)
-  __shared__ T s_l[kDB * P];   // L^-1, s_l[i*P + j]
-  __shared__ T s_w[kDB * P];   // U^-1
-  for (int e = threadIdx.x; e < kDB * kDB; e += kThreads) {
+  // s_a[k*kDB + i] = A(r0+i, r0+k); the identity outside the nr x nr corner
+  __shared__ __attribute__((aligned(16))) T s_a[kDB * kDB];
+  for (int e = threadIdx.x; e < kDB * kDB; e += 128) {
     const int i = e % kDB, k = e / kDB;
-    s_a[i * P + k] = (i < nr && k < nr) ? A[(r0 + i) + int64_t(r0 + k) * d.lda] : T(0);
-    s_l[i * P + k] = T(0);
-    s_w[i * P + k] = T(0);
+    s_a[e] = (i < nr && k < nr) ? A[(r0 + i) + int64_t(k + r0) * d.lda] : (i == k ? T(1) : T(0));
   }
   __syncthreads();
-  const int tid = threadIdx.x;
-  if (tid < nr) {
-    const int j = tid;
-    s_l[j * P + j] = T(1);
-    for (int i = j + 1; i < nr; ++i) {
-      T x = 0;
-      for (int k = j; k < i; ++k) x -= s_a[i * P + k] * s_l[k * P + j];
-      s_l[i * P + j] = x;
+  const int j = threadIdx.x & 63;
+  T x[kDB];
+#pragma unroll
+  for (int i = 0; i < kDB; ++i) x[i] = i == j ? T(1) : T(0);
+  if (threadIdx.x < 64) {
+    // L x = e_j, forward: x_k is final at step k
+#pragma unroll
+    for (int k = 0; k < kDB - 1; ++k) {
+#pragma unroll
+      for (int i = k + 1; i < kDB; ++i) {
+        x[i] -= s_a[k * kDB + i] * x[k];
+        if ((i - k) % 16 == 0) __builtin_amdgcn_sched_barrier(0);   // at most 16 entries in flight
+      }
     }
-  } else if (tid >= kDB && tid < kDB + nr) {
-    const int j = tid - kDB;
-    s_w[j * P + j] = T(1) / s_a[j * P + j];
-    for (int i = j - 1; i >= 0; --i) {
-      T x = 0;
-      for (int k = i + 1; k <= j; ++k) x -= s_a[i * P + k] * s_w[k * P + j];
-      s_w[i * P + j] = x / s_a[i * P + i];
+  } else {
+    // U x = e_j, backward; x_k stays 0 for k > j, where the divisor is 1 so a
+    // zero pivot outside the column's range does not turn it into NaN
+#pragma unroll
+    for (int k = kDB - 1; k >= 0; --k) {
+      x[k] = x[k] / (k <= j ? s_a[k * kDB + k] : T(1));
+#pragma unroll
+      for (int i = 0; i < k; ++i) {
+        x[i] -= s_a[k * kDB + i] * x[k];
+        if (i % 16 == 15) __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
-  __syncthreads();
-  HODLR_GLOBAL T* out = glob((T*)d.dinv + int64_t(b) * 2 * kDB * kDB);
-  for (int e = threadIdx.x; e < kDB * kDB; e += kThreads) {
-    const int i = e % kDB, j = e / kDB;
-    out[e] = s_l[i * P + j];
-    out[kDB * kDB + e] = s_w[i * P + j];
-  }
+  // column j of the inverse, contiguous in the column-major output (ld kDB)
+  HODLR_GLOBAL T* out = glob((T*)d.dinv + int64_t(b) * 2 * kDB * kDB + (threadIdx.x < 64 ? 0 : kDB * kDB) +
+                             int64_t(j) * kDB);
+  const bool live = j < nr;
+#pragma unroll
+  for (int i = 0; i < kDB; ++i) out[i] = live ? x[i] : T(0);
 }
 
 // ======================================================================= TRSM
@@ -761,7 +730,9 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
     HODLR_GEMM3(2, float, double, float, float, false);
   } else if (ta == F32 && tb == F32 && tc == F32 && mt == F32 && !trans) {
     if (role == 2) HODLR_GEMM3(2, float, float, float, float, false);
-    else HODLR_GEMM3(3, float, float, float, float, false);
+    else if (role == 3) HODLR_GEMM3(3, float, float, float, float, false);
+    else if (tile == 2) HODLR_GEMM(0, float, float, float, float, false, 64, kGemmNarrowBN);
+    else HODLR_GEMM(0, float, float, float, float, false, 64, 64);
   } else if (ta == F32 && tb == F32 && tc == F64 && mt == F32 && trans) {
     HODLR_GEMM3(1, float, float, double, float, true);
   } else if (ta == F32 && tb == F32 && tc == F64 && mt == BF16 && trans) {
@@ -813,20 +784,11 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
   return hipGetLastError();
 }
 
-hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  dim3 g((mmax + kDB - 1) / kDB, n), b(kThreads);
-  if (t == F64) hipLaunchKernelGGL((lu_dinv_kernel<double>), g, b, 0, s, d);
-  else hipLaunchKernelGGL((lu_dinv_kernel<float>), g, b, 0, s, d);
-  return hipGetLastError();
-}
-
-hipError_t launch_lu_rowblock(DT t, const LuDesc* d, int n, int k0, int mmax, hipStream_t s) {
-  const int ncols = mmax - k0 - kLuNB;
-  if (n <= 0 || ncols <= 0) return hipSuccess;
-  dim3 g((ncols + 63) / 64, n), b(kThreads);
-  if (t == F64) hipLaunchKernelGGL((lu_rowblock_kernel<double>), g, b, 0, s, d, k0);
-  else hipLaunchKernelGGL((lu_rowblock_kernel<float>), g, b, 0, s, d, k0);
+hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStream_t s) {
+  if (n <= 0 || nblk <= 0) return hipSuccess;
+  dim3 g(nblk, n), b(128);
+  if (t == F64) hipLaunchKernelGGL((lu_dinv_kernel<double>), g, b, 0, s, d, b0);
+  else hipLaunchKernelGGL((lu_dinv_kernel<float>), g, b, 0, s, d, b0);
   return hipGetLastError();
 }
 

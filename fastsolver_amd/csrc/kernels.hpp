@@ -30,9 +30,9 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int t
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s);  // double partials
 hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s);
-hipError_t launch_lu_rowblock(DT t, const LuDesc* d, int n, int k0, int mmax, hipStream_t s);
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
-hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int mmax, hipStream_t s);  // inverted 64x64 diagonal blocks
+// inverted 64 x 64 diagonal blocks b0 .. b0+nblk-1 of every factor
+hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStream_t s);
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
 // (TLU, TIN, TOUT) in {(F64,F64,F64), (F32,F64,F32), (F32,F32,F32)}; compute type = TLU, except the
 // products with the solved rows, which run in `ut` (= TLU, or BF16 with TLU = F32, variant 0 only)

@@ -48,7 +48,7 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
                                 "solve_bcast"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
-enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_ROW, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN };
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN };
 
 struct Launch {
   Kind kind;
@@ -183,26 +183,16 @@ struct Builder {
             int64_t min_wg = 0) {
     std::vector<GemmDesc> d;
     std::vector<SumDesc> sums;
-    // tile size: 128 x 128 for the big roles when the problems are large enough
-    // and the batch still fills the GPU; launches of roles 1-3 carry both
-    // tilings and the plan's autotuner may switch them
-    // launches whose problems are all at most 32 columns wide (the residual
-    // and correction sweeps on c <= 32 right-hand sides) use 64 x 32 tiles
+    // tile size: 64 x 64 (measured the faster one for nearly every launch of
+    // the cfg-family workloads); launches of roles 1-3 also carry the 128 x 128
+    // tiling, which the plan's autotuner may switch to.  Launches whose
+    // problems are all at most 32 columns wide (the residual and correction
+    // sweeps on c <= 32 right-hand sides) use 64 x 32 tiles.
     int tile = 0;
     bool narrow = true;
-    {
-      double sm = 0, sn = 0;
-      int cnt = 0;
-      int64_t big_tiles = 0;
-      for (const G& g : gs)
-        if (g.m > 0 && g.n > 0) {
-          sm += g.m; sn += g.n; ++cnt; big_tiles += cdiv(g.m, 128) * cdiv(g.n, 128);
-          narrow = narrow && g.n <= kGemmNarrowBN;
-        }
-      if (role >= 1 && mt != F64 && cnt && sm / cnt >= 96 && sn / cnt >= 96 && (big_tiles >= 512 || min_wg > 0))
-        tile = 1;
-      if (narrow) tile = 2;
-    }
+    for (const G& g : gs)
+      if (g.m > 0 && g.n > 0) narrow = narrow && g.n <= kGemmNarrowBN;
+    if (narrow) tile = 2;
     double work = 0;   // 64-tile rows of K work (independent of the tiling)
     for (const G& g : gs)
       if (g.m > 0 && g.n > 0) work += double(cdiv(g.m, kGemmBM) * cdiv(g.n, kGemmBN)) * std::max(g.k, 1);
@@ -320,17 +310,37 @@ struct Builder {
         if (x.m > k0) { int nb = std::min(kLuNB, x.m - k0); pf += double(x.m - k0) * nb * nb; }
       Pn.flops = pf;
       pr.l.push_back(Pn);
+      // the panel's diagonal block is final: invert it (the TRSMs use it too)
+      Launch Dv;
+      Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.k0 = k0;
+      pr.l.push_back(Dv);
       if (mmax - k0 - kLuNB > 0) {
-        Launch R;
-        R.kind = K_LU_ROW; R.cat = row_cat; R.t[0] = t; R.off = off; R.count = int(v.size());
-        R.k0 = k0; R.mmax = mmax;
-        pr.l.push_back(R);
+        const size_t es = esize(t);
+        // U12 = L11^-1 A12 in place: one 64-row tile per column block, so every
+        // workgroup reads its A12 columns before it overwrites them
+        std::vector<G> rs;
+        for (auto& x : v) {
+          int rest = x.m - k0 - kLuNB;
+          if (rest <= 0) continue;
+          char* A = (char*)x.A;
+          G g;
+          g.A = (char*)x.dinv + es * size_t(k0 / kLuNB) * 2 * kLuNB * kLuNB;
+          g.lda = kLuNB;
+          g.B = A + es * (size_t(k0) + size_t(k0 + kLuNB) * x.lda);
+          g.ldb = x.lda;
+          g.Cout = (void*)g.B;
+          g.Cin = g.Cout;
+          g.ldcin = g.ldcout = x.lda;
+          g.m = kLuNB; g.n = rest; g.k = kLuNB;
+          g.alpha = 1.0; g.beta = 0.0;
+          rs.push_back(g);
+        }
+        gemm(pr, row_cat, t, t, t, t, false, 0, rs);
         std::vector<G> gs;
         for (auto& x : v) {
           int rest = x.m - k0 - kLuNB;
           if (rest <= 0) continue;
           char* A = (char*)x.A;
-          size_t es = esize(t);
           G g;
           g.A = A + es * (size_t(k0 + kLuNB) + size_t(k0) * x.lda);
           g.lda = x.lda;
@@ -349,9 +359,6 @@ struct Builder {
     Launch Pm;
     Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
     pr.l.push_back(Pm);
-    Launch Dv;
-    Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.mmax = mmax;
-    pr.l.push_back(Dv);
   }
 
   // Both TRSM variants (64-row steps / 128-row steps) get descriptor sets;
@@ -380,7 +387,7 @@ struct Builder {
       L.off_t[var] = blob.add(d);
       L.tiles_t[var] = t;
     }
-    L.tile = both ? 1 : 0;
+    L.tile = 0;   // 64-row steps: the faster variant on every measured workload
     if (tunable && both) L.gid = int(ngid++);
     L.off = L.off_t[L.tile];
     L.tiles = L.tiles_t[L.tile];
@@ -773,12 +780,11 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
       return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile,
                          (const GemmDesc*)(P->descs.p + L.off_t[tile]), L.count, L.tiles_t[tile], s);
     }
-    case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.mmax, s);
+    case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, 1, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
-    case K_LU_ROW: return launch_lu_rowblock(L.t[0], (const LuDesc*)d, L.count, L.k0, L.mmax, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
     case K_TRSM: {
       int var = L.tile;
