@@ -24,12 +24,14 @@ struct GemmDesc {
 };
 
 // out = sum_{s < nsplit} P[s*stride + i + j*m]   (fixed order), m x n
+// out = sum over q < nsplit of the m x n partial at P + q*stride (leading
+// dimension ldp), added in q order.
 struct SumDesc {
   const void* P;
   int64_t stride;
   void* out;
   int64_t ldo;
-  int32_t m, n, nsplit, pad0;
+  int32_t m, n, nsplit, ldp;
   int64_t tile_begin;
 };
 

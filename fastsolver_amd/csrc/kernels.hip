@@ -114,9 +114,11 @@ __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict
   for (int e = threadIdx.x; e < kSumTile; e += kThreads) {
     int64_t idx = base + e;
     if (idx >= total) break;
+    const int64_t i = idx % d.m, j = idx / d.m;
+    const HODLR_GLOBAL double* src = P + i + j * d.ldp;
     double s = 0.0;
-    for (int q = 0; q < d.nsplit; ++q) s += P[q * d.stride + idx];
-    int64_t i = idx % d.m, j = idx / d.m;
+#pragma unroll 8
+    for (int q = 0; q < d.nsplit; ++q) s += src[q * d.stride];
     out[i + j * d.ldo] = s;
   }
 }

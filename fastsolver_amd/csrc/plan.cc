@@ -93,6 +93,9 @@ struct SysState {
   std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
   std::vector<size_t> z_off;       // per leaf (doubles)
   std::vector<size_t> vtd, vtu, wmv;   // per node (doubles): reduce outputs
+  std::vector<int> leaf_lo, leaf_hi;   // per node: its leaves are leaf_id leaf_lo .. leaf_hi-1
+  size_t pmv = 0;                  // matvec: per leaf V_j^T x_j, pld x c each (doubles)
+  int64_t pld = 0;
   std::vector<size_t> sbuf, eta, spiv, sperm, sdinv;  // per interior node
   std::vector<size_t> etaf;        // per interior node, FP32 plans: eta0 | eta1 in the workspace type
   std::vector<int> sinfo;          // per interior node: index into node info array
@@ -223,7 +226,7 @@ struct Builder {
         sd.stride = x.split_stride;
         sd.out = g.Cout;
         sd.ldo = g.ldcout;
-        sd.m = g.m; sd.n = g.n; sd.nsplit = nsplit;
+        sd.m = g.m; sd.n = g.n; sd.nsplit = nsplit; sd.ldp = g.m;
         sums.push_back(sd);
         part += size_t(nsplit) * g.m * g.n;
       }
@@ -262,14 +265,23 @@ struct Builder {
     L.off = L.off_t[tile];
     L.tiles = L.tiles_t[tile];
     pr.l.push_back(L);
-    if (!sums.empty()) {
-      int64_t st = 0;
-      for (auto& s : sums) { s.tile_begin = st; st += cdiv(int64_t(s.m) * s.n, kSumTile); }
-      Launch S;
-      S.kind = K_SUM; S.cat = cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat;
-      S.off = blob.add(sums); S.count = int(sums.size()); S.tiles = st;
-      pr.l.push_back(S);
+    sum(pr, cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat, sums);
+  }
+
+  void sum(Proc& pr, Cat cat, std::vector<SumDesc> v) {
+    std::vector<SumDesc> d;
+    int64_t st = 0;
+    for (auto& s : v) {
+      if (s.m <= 0 || s.n <= 0 || s.nsplit <= 0) continue;
+      s.tile_begin = st;
+      st += cdiv(int64_t(s.m) * s.n, kSumTile);
+      d.push_back(s);
     }
+    if (d.empty()) return;
+    Launch S;
+    S.kind = K_SUM; S.cat = cat;
+    S.off = blob.add(d); S.count = int(d.size()); S.tiles = st;
+    pr.l.push_back(S);
   }
 
   void copy(Proc& pr, Cat cat, DT ts, DT td, std::vector<CopyDesc> v) {
@@ -462,6 +474,18 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     S.ldinv = L.take(dinv_elems * we);
     S.zbuf = L.take(std::max<size_t>(z_elems, 1) * 8);
     const size_t nn = t.nodes.size();
+    S.leaf_lo.assign(nn, int(t.leaves.size()));
+    S.leaf_hi.assign(nn, 0);
+    S.pld = 0;
+    for (size_t j = 0; j < t.leaves.size(); ++j) {
+      S.pld = std::max<int64_t>(S.pld, t.rsum[t.nodes[t.leaves[j]].depth]);
+      for (int a = t.leaves[j];; a = t.nodes[a].parent) {
+        S.leaf_lo[a] = std::min(S.leaf_lo[a], int(j));
+        S.leaf_hi[a] = std::max(S.leaf_hi[a], int(j) + 1);
+        if (a == 0) break;
+      }
+    }
+    S.pmv = L.take(std::max<size_t>(t.leaves.size() * size_t(S.pld) * c, 1) * 8);
     S.vtd.assign(nn, 0); S.vtu.assign(nn, 0); S.wmv.assign(nn, 0);
     S.sbuf.assign(nn, 0); S.eta.assign(nn, 0); S.spiv.assign(nn, 0); S.sperm.assign(nn, 0); S.sdinv.assign(nn, 0);
     S.etaf.assign(nn, 0);
@@ -492,6 +516,40 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
   }
   P->nnodes = node_count;
   return 0;
+}
+
+// Matvec reduce: W_i = V_i^T x_i for every non-root node i (V_i: the rows of
+// node i in the column block of its depth), for the HODLR product of the
+// residual.  One product per leaf reads each V row block once,
+// P_j = V(rows of leaf j, 0 : R_D)^T x_j, and W_i is then the sum, in leaf
+// order, of the depth block of P_j over the leaves j under node i.
+void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, std::vector<Builder::G>& prod,
+                   std::vector<SumDesc>& sums) {
+  const Tree& t = S.tree;
+  const int64_t c = t.nrhs;
+  double* Pb = at<double>(P, S.pmv);
+  const int64_t stride = S.pld * c;
+  for (size_t j = 0; j < t.leaves.size(); ++j) {
+    const TreeNode& nd = t.nodes[t.leaves[j]];
+    const int64_t Rd = t.rsum[nd.depth];
+    if (Rd == 0) continue;
+    Builder::G g{};
+    g.A = S.s.V + nd.row_beg; g.lda = S.s.ldv;
+    g.B = x + nd.row_beg; g.ldb = ldx;
+    g.Cout = Pb + int64_t(j) * stride; g.ldcout = S.pld;
+    g.m = int(Rd); g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
+    prod.push_back(g);
+  }
+  for (size_t i = 1; i < t.nodes.size(); ++i) {
+    const TreeNode& nd = t.nodes[i];
+    const int r = t.rank(nd.depth);
+    SumDesc sd{};
+    sd.P = Pb + int64_t(S.leaf_lo[i]) * stride + t.rsum[nd.depth - 1];
+    sd.stride = stride; sd.ldp = int(S.pld);
+    sd.out = at<double>(P, S.wmv[i]); sd.ldo = r;
+    sd.m = r; sd.n = int(c); sd.nsplit = S.leaf_hi[i] - S.leaf_lo[i];
+    sums.push_back(sd);
+  }
 }
 
 void build_all(hodlr_plan_s* P, Builder& B) {
@@ -702,6 +760,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
   // ---------------------------------------------------------------- residual r = b - A x
   {
     std::vector<Builder::G> red, lk, lu;
+    std::vector<SumDesc> wsum;
     std::vector<CopyDesc> gz;
     std::vector<NormDesc> nr;
     for (SysState& S : P->sys) {
@@ -709,16 +768,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       const int64_t c = t.nrhs;
       double* R = at<double>(P, S.rbuf);
       double* Z = at<double>(P, S.zbuf);
-      for (size_t i = 1; i < t.nodes.size(); ++i) {
-        const TreeNode& x = t.nodes[i];
-        const int r = t.rank(x.depth);
-        Builder::G g{};
-        g.A = S.s.V + x.row_beg + t.rsum[x.depth - 1] * S.s.ldv; g.lda = S.s.ldv;
-        g.B = S.s.x + x.row_beg; g.ldb = S.s.ldxs;
-        g.Cout = at<double>(P, S.wmv[i]); g.ldcout = r;
-        g.m = r; g.n = int(c); g.k = int(x.nrow); g.alpha = 1.0; g.beta = 0.0;
-        red.push_back(g);
-      }
+      matvec_reduce(P, S, S.s.x, S.s.ldxs, red, wsum);
       for (size_t j = 0; j < t.leaves.size(); ++j) {
         const int id = t.leaves[j];
         const TreeNode& x = t.nodes[id];
@@ -758,7 +808,8 @@ void build_all(hodlr_plan_s* P, Builder& B) {
         nr.push_back(d);
       }
     }
-    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, true, 1, red, 4096);
+    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
+    B.sum(P->residual, CAT_MATVEC, wsum);
     B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
     B.copy(P->residual, CAT_MATVEC, F64, F64, gz);
     B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
@@ -1336,22 +1387,14 @@ int hodlr_matvec(hodlr_plan_t P, const double* const* x, const int64_t* ldx, dou
   Builder B{P};
   Proc pr;
   std::vector<Builder::G> red, lk, lu;
+  std::vector<SumDesc> wsum;
   std::vector<CopyDesc> gz;
   for (size_t s = 0; s < P->sys.size(); ++s) {
     SysState& S = P->sys[s];
     const Tree& t = S.tree;
     const int64_t c = t.nrhs;
     double* Z = at<double>(P, S.zbuf);
-    for (size_t i = 1; i < t.nodes.size(); ++i) {
-      const TreeNode& nd = t.nodes[i];
-      const int r = t.rank(nd.depth);
-      Builder::G g{};
-      g.A = S.s.V + nd.row_beg + t.rsum[nd.depth - 1] * S.s.ldv; g.lda = S.s.ldv;
-      g.B = x[s] + nd.row_beg; g.ldb = ldx[s];
-      g.Cout = at<double>(P, S.wmv[i]); g.ldcout = r;
-      g.m = r; g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
-      red.push_back(g);
-    }
+    matvec_reduce(P, S, x[s], ldx[s], red, wsum);
     for (size_t j = 0; j < t.leaves.size(); ++j) {
       const int id = t.leaves[j];
       const TreeNode& nd = t.nodes[id];
@@ -1384,7 +1427,8 @@ int hodlr_matvec(hodlr_plan_t P, const double* const* x, const int64_t* ldx, dou
       lu.push_back(u);
     }
   }
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red, 4096);
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
+  B.sum(pr, CAT_MATVEC, wsum);
   B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
   B.copy(pr, CAT_MATVEC, F64, F64, gz);
   B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
