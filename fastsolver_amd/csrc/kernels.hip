@@ -145,15 +145,18 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 
 // ========================================================================= LU
 // Panel of kLuNB columns starting at k0, factored as kLuNB/PW sub-panels of PW
-// columns held in registers: NT threads (128, 256, 512 or 1024, the smallest that
-// covers the batch's largest block), thread t owns row ks + t of the sub-panel.  PW = 16 for fp64 and 32 for fp32
-// keeps the register panel within the 128 VGPRs a 1024-thread group allows.
+// columns held in registers: 256 threads, thread t owns rows ks + t + q*256
+// (q < RPT) of the sub-panel, RPT = 1, 2 or 4 (the smallest that covers the
+// batch's largest block).  A 256-thread group keeps the column steps' barriers
+// cheap and lets two blocks share a CU, so one block's pivot latency hides
+// behind the other's.  PW = 16 (fp64, or fp32 with RPT = 4) or 32 (fp32).
 // Partial pivoting picks the first largest |a| (idamax); the column below a
 // zero pivot is left unscaled and info records it, as dgetf2 does.
-constexpr int kPanelThreads = 1024;   // largest block the panel kernel holds
+constexpr int kPanelNT = 256;
+constexpr int kPanelThreads = 4 * kPanelNT;   // largest block the panel kernel holds
 
-template <class T> struct PanelWidth { static constexpr int value = 16; };
-template <> struct PanelWidth<float> { static constexpr int value = 32; };
+// sub-panel width: a[RPT][PW] stays within 128 registers
+template <class T, int RPT> struct PanelWidth { static constexpr int value = sizeof(T) == 4 && RPT < 4 ? 32 : 16; };
 
 template <class T>
 __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
@@ -208,12 +211,13 @@ __device__ __forceinline__ void swap_rows_in_column(PT A, int64_t col, int r1, i
   }
 }
 
-template <class T, int NT>
-__global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
-  constexpr int kPanelWaves = NT / 64;
-  constexpr int PW = PanelWidth<T>::value;
+template <class T, int RPT>
+__global__ __launch_bounds__(kPanelNT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+  constexpr int NT = kPanelNT, NW = NT / 64;
+  constexpr int PW = PanelWidth<T, RPT>::value;
   constexpr int NSP = kLuNB / PW;
   constexpr int PX = kLuNB + 1;
+  static_assert(PW <= NT, "row j of a sub-panel is row j of thread j");
   const LuDesc d = descs[blockIdx.x];
   const int m = d.m;
   if (k0 >= m) return;
@@ -224,8 +228,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
 
   __shared__ T s_rowp[2][PW];
   __shared__ T s_rowj[2][PW];
-  __shared__ T s_rv[2][kPanelWaves];
-  __shared__ int s_ri[2][kPanelWaves];
+  __shared__ T s_rv[2][NW];
+  __shared__ int s_ri[2][NW];
   __shared__ int s_piv[kLuNB];        // absolute pivot row of panel column jg
   __shared__ T s_l[PW * (PW + 1)];    // L11 of the current sub-panel
   __shared__ T s_x[PW * PX];          // U12 rows of the current sub-panel
@@ -238,32 +242,44 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
     if (ks >= k0 + nb) break;
     const int w = min(PW, k0 + nb - ks);
     const int rows = m - ks;
-    const bool mine = tid < rows;
-    T a[PW];
+    T a[RPT][PW];
 #pragma unroll
-    for (int jj = 0; jj < PW; ++jj) a[jj] = (mine && jj < w) ? A[(ks + tid) + int64_t(ks + jj) * lda] : T(0);
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NT;
+#pragma unroll
+      for (int jj = 0; jj < PW; ++jj) a[q][jj] = (r < rows && jj < w) ? A[(ks + r) + int64_t(ks + jj) * lda] : T(0);
+    }
 #pragma unroll
     for (int j = 0; j < PW; ++j) {
       if (j < w) {
         const int buf = j & 1;
-        T bv = (mine && tid >= j) ? T(fabs(a[j])) : T(-1);
-        int bi = (mine && tid >= j) ? tid : 0x7fffffff;
-        if (!(bv == bv)) { bv = T(-1); bi = 0x7fffffff; }   // NaN rows never win
+        // first largest |a| of the thread's rows at or below j (NaN never wins)
+        T bv = T(-1);
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * NT;
+          const T v = fabs(a[q][j]);
+          if (r < rows && r >= j && v > bv) { bv = v; bi = r; }
+        }
         wave_argmax(bv, bi);
         if (lane == 0) { s_rv[buf][wave] = bv; s_ri[buf][wave] = bi; }
         __syncthreads();
         T pv = s_rv[buf][0];
         int p = s_ri[buf][0];
 #pragma unroll
-        for (int q = 1; q < kPanelWaves; ++q) pivot_combine(pv, p, s_rv[buf][q], s_ri[buf][q]);
+        for (int q = 1; q < NW; ++q) pivot_combine(pv, p, s_rv[buf][q], s_ri[buf][q]);
         if (p >= rows) p = j;   // no candidate (all NaN): keep row j
-        if (tid == p) {
+        // (row tests rather than a computed q keep a[][] statically indexed)
 #pragma unroll
-          for (int jj = 0; jj < PW; ++jj) s_rowp[buf][jj] = a[jj];
-        }
+        for (int q = 0; q < RPT; ++q)
+          if (tid + q * NT == p) {
+#pragma unroll
+            for (int jj = 0; jj < PW; ++jj) s_rowp[buf][jj] = a[q][jj];
+          }
         if (tid == j) {
 #pragma unroll
-          for (int jj = 0; jj < PW; ++jj) s_rowj[buf][jj] = a[jj];
+          for (int jj = 0; jj < PW; ++jj) s_rowj[buf][jj] = a[0][jj];
         }
         if (tid == 0) {
           s_piv[PW * sp + j] = ks + p;
@@ -274,35 +290,47 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
         if (p != j) {
           if (tid == j) {
 #pragma unroll
-            for (int jj = 0; jj < PW; ++jj) a[jj] = s_rowp[buf][jj];
-          } else if (tid == p) {
-#pragma unroll
-            for (int jj = 0; jj < PW; ++jj) a[jj] = s_rowj[buf][jj];
+            for (int jj = 0; jj < PW; ++jj) a[0][jj] = s_rowp[buf][jj];
           }
+#pragma unroll
+          for (int q = 0; q < RPT; ++q)
+            if (tid + q * NT == p) {
+#pragma unroll
+              for (int jj = 0; jj < PW; ++jj) a[q][jj] = s_rowj[buf][jj];
+            }
         }
         const T piv = s_rowp[buf][j];
-        if (mine && tid > j) {
-          const T l = piv != T(0) ? a[j] / piv : a[j];
-          a[j] = l;
 #pragma unroll
-          for (int jj = j + 1; jj < PW; ++jj) a[jj] -= l * s_rowp[buf][jj];
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * NT;
+          if (r < rows && r > j) {
+            const T l = piv != T(0) ? a[q][j] / piv : a[q][j];
+            a[q][j] = l;
+#pragma unroll
+            for (int jj = 0; jj < PW; ++jj)   // fixed trip count: unrolled before the j loop
+              if (jj > j) a[q][jj] -= l * s_rowp[buf][jj];
+          }
         }
       }
     }
-    if (mine) {
 #pragma unroll
-      for (int jj = 0; jj < PW; ++jj)
-        if (jj < w) A[(ks + tid) + int64_t(ks + jj) * lda] = a[jj];
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NT;
+      if (r < rows) {
+#pragma unroll
+        for (int jj = 0; jj < PW; ++jj)
+          if (jj < w) A[(ks + r) + int64_t(ks + jj) * lda] = a[q][jj];
+      }
     }
     if (tid < PW) {
 #pragma unroll
-      for (int jj = 0; jj < PW; ++jj) s_l[tid * (PW + 1) + jj] = a[jj];
+      for (int jj = 0; jj < PW; ++jj) s_l[tid * (PW + 1) + jj] = a[0][jj];
     }
     __syncthreads();
     // this sub-panel's interchanges on the other panel columns, in order
     const int nleft = ks - k0, nright = k0 + nb - (ks + PW);
-    if (tid < nleft + max(nright, 0)) {
-      const int col = tid < nleft ? k0 + tid : ks + PW + (tid - nleft);
+    for (int t = tid; t < nleft + max(nright, 0); t += NT) {
+      const int col = t < nleft ? k0 + t : ks + PW + (t - nleft);
       const int64_t cb = int64_t(col) * lda;
       for (int j = 0; j < w; ++j) swap_rows_in_column(A, cb, ks + j, s_piv[PW * sp + j]);
     }
@@ -326,13 +354,25 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
         const int r = e % PW, c = e / PW;
         A[(ks + r) + int64_t(ks + PW + c) * lda] = s_x[r * PX + c];
       }
-      // A22 -= L21 U12 on the remaining panel columns, L21 rows from registers
-      if (mine && tid >= PW) {
-        for (int c = 0; c < nright; ++c) {
-          T x = A[(ks + tid) + int64_t(ks + PW + c) * lda];
+      // A22 -= L21 U12 on the remaining panel columns, L21 rows from registers;
+      // the thread's RPT loads of a column are issued together
+      for (int c = 0; c < nright; ++c) {
+        T x[RPT];
 #pragma unroll
-          for (int k = 0; k < PW; ++k) x -= a[k] * s_x[k * PX + c];
-          A[(ks + tid) + int64_t(ks + PW + c) * lda] = x;
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * NT;
+          x[q] = (r >= PW && r < rows) ? A[(ks + r) + int64_t(ks + PW + c) * lda] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < PW; ++k) {
+          const T u = s_x[k * PX + c];
+#pragma unroll
+          for (int q = 0; q < RPT; ++q) x[q] -= a[q][k] * u;
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * NT;
+          if (r >= PW && r < rows) A[(ks + r) + int64_t(ks + PW + c) * lda] = x[q];
         }
       }
       __syncthreads();
@@ -772,16 +812,15 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (rows_max > kPanelThreads) return hipErrorInvalidValue;
-  dim3 g(n);
-#define HODLR_PANEL(NT)                                                                    \
-  do {                                                                                     \
-    if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT>), g, dim3(NT), 0, s, d, k0); \
-    else hipLaunchKernelGGL((lu_panel_kernel<float, NT>), g, dim3(NT), 0, s, d, k0);         \
+  dim3 g(n), b(kPanelNT);
+#define HODLR_PANEL(RPT)                                                                     \
+  do {                                                                                       \
+    if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, RPT>), g, b, 0, s, d, k0);    \
+    else hipLaunchKernelGGL((lu_panel_kernel<float, RPT>), g, b, 0, s, d, k0);              \
   } while (0)
-  if (rows_max <= 128) HODLR_PANEL(128);
-  else if (rows_max <= 256) HODLR_PANEL(256);
-  else if (rows_max <= 512) HODLR_PANEL(512);
-  else HODLR_PANEL(1024);
+  if (rows_max <= kPanelNT) HODLR_PANEL(1);
+  else if (rows_max <= 2 * kPanelNT) HODLR_PANEL(2);
+  else HODLR_PANEL(4);
 #undef HODLR_PANEL
   return hipGetLastError();
 }
