@@ -354,25 +354,43 @@ __global__ __launch_bounds__(kPanelNT) void lu_panel_kernel(const LuDesc* __rest
         const int r = e % PW, c = e / PW;
         A[(ks + r) + int64_t(ks + PW + c) * lda] = s_x[r * PX + c];
       }
-      // A22 -= L21 U12 on the remaining panel columns, L21 rows from registers;
-      // the thread's RPT loads of a column are issued together
-      for (int c = 0; c < nright; ++c) {
-        T x[RPT];
+      // A22 -= L21 U12 on the remaining panel columns, L21 rows from registers.
+      // The column loads run PD columns ahead of the updates (a column past
+      // nright reloads the last one and is never stored), so a thread waits for
+      // memory about once per PD columns rather than once per column.
+      constexpr int PD = sizeof(T) == 8 && RPT == 4 ? 2 : 4;
+      T pre[PD][RPT];
+      auto load_col = [&](T* dst, int c) {
+        const int64_t cb = int64_t(ks + PW + min(c, nright - 1)) * lda;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
           const int r = tid + q * NT;
-          x[q] = (r >= PW && r < rows) ? A[(ks + r) + int64_t(ks + PW + c) * lda] : T(0);
+          dst[q] = (r >= PW && r < rows) ? A[(ks + r) + cb] : T(0);
         }
+      };
 #pragma unroll
-        for (int k = 0; k < PW; ++k) {
-          const T u = s_x[k * PX + c];
+      for (int u = 0; u < PD; ++u) load_col(pre[u], u);
+      for (int c0 = 0; c0 < nright; c0 += PD) {
 #pragma unroll
-          for (int q = 0; q < RPT; ++q) x[q] -= a[q][k] * u;
-        }
+        for (int u = 0; u < PD; ++u) {
+          const int c = c0 + u;
+          T x[RPT];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          const int r = tid + q * NT;
-          if (r >= PW && r < rows) A[(ks + r) + int64_t(ks + PW + c) * lda] = x[q];
+          for (int q = 0; q < RPT; ++q) x[q] = pre[u][q];
+          load_col(pre[u], c + PD);
+          if (c < nright) {
+#pragma unroll
+            for (int k = 0; k < PW; ++k) {
+              const T uv = s_x[k * PX + c];
+#pragma unroll
+              for (int q = 0; q < RPT; ++q) x[q] -= a[q][k] * uv;
+            }
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+              const int r = tid + q * NT;
+              if (r >= PW && r < rows) A[(ks + r) + int64_t(ks + PW + c) * lda] = x[q];
+            }
+          }
         }
       }
       __syncthreads();
