@@ -313,60 +313,83 @@ struct Builder {
     Launch I;
     I.kind = K_LU_INIT; I.cat = misc_cat; I.off = off; I.count = int(v.size());
     pr.l.push_back(I);
-    for (int k0 = 0; k0 < mmax; k0 += kLuNB) {
+    const size_t es = esize(t);
+    // factor the 64 columns at kp and invert their diagonal block (final once
+    // the panel is done; the row solves and the TRSMs use it)
+    auto panel = [&](int kp) {
       Launch Pn;
       Pn.kind = K_LU_PANEL; Pn.cat = panel_cat; Pn.t[0] = t; Pn.off = off; Pn.count = int(v.size());
-      Pn.k0 = k0; Pn.rows_max = mmax - k0;
+      Pn.k0 = kp; Pn.rows_max = mmax - kp;
       double pf = 0;
       for (auto& x : v)
-        if (x.m > k0) { int nb = std::min(kLuNB, x.m - k0); pf += double(x.m - k0) * nb * nb; }
+        if (x.m > kp) { int nb = std::min(kLuNB, x.m - kp); pf += double(x.m - kp) * nb * nb; }
       Pn.flops = pf;
       pr.l.push_back(Pn);
-      // the panel's diagonal block is final: invert it (the TRSMs use it too)
       Launch Dv;
-      Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.k0 = k0;
+      Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.k0 = kp;
       pr.l.push_back(Dv);
-      if (mmax - k0 - kLuNB > 0) {
-        const size_t es = esize(t);
-        // U12 = L11^-1 A12 in place: one 64-row tile per column block, so every
-        // workgroup reads its A12 columns before it overwrites them
-        std::vector<G> rs;
-        for (auto& x : v) {
-          int rest = x.m - k0 - kLuNB;
-          if (rest <= 0) continue;
-          char* A = (char*)x.A;
-          G g;
-          g.A = (char*)x.dinv + es * size_t(k0 / kLuNB) * 2 * kLuNB * kLuNB;
-          g.lda = kLuNB;
-          g.B = A + es * (size_t(k0) + size_t(k0 + kLuNB) * x.lda);
-          g.ldb = x.lda;
-          g.Cout = (void*)g.B;
-          g.Cin = g.Cout;
-          g.ldcin = g.ldcout = x.lda;
-          g.m = kLuNB; g.n = rest; g.k = kLuNB;
-          g.alpha = 1.0; g.beta = 0.0;
-          rs.push_back(g);
-        }
-        gemm(pr, row_cat, t, t, t, t, false, 0, rs);
-        std::vector<G> gs;
-        for (auto& x : v) {
-          int rest = x.m - k0 - kLuNB;
-          if (rest <= 0) continue;
-          char* A = (char*)x.A;
-          G g;
-          g.A = A + es * (size_t(k0 + kLuNB) + size_t(k0) * x.lda);
-          g.lda = x.lda;
-          g.B = A + es * (size_t(k0) + size_t(k0 + kLuNB) * x.lda);
-          g.ldb = x.lda;
-          g.Cout = A + es * (size_t(k0 + kLuNB) + size_t(k0 + kLuNB) * x.lda);
-          g.Cin = g.Cout;
-          g.ldcin = g.ldcout = x.lda;
-          g.m = rest; g.n = rest; g.k = kLuNB;
-          g.alpha = -1.0; g.beta = 1.0;
-          gs.push_back(g);
-        }
-        gemm(pr, upd_cat, t, t, t, upd_mt, false, 3, gs);
+    };
+    // U12 = L_kp^-1 A(kp : kp+64, c0 : c0+n) in place, n = min(cmax, m - c0):
+    // one 64-row tile per column block, so every workgroup reads its columns
+    // before it overwrites them
+    auto row_solve = [&](int kp, int c0, int cmax) {
+      std::vector<G> rs;
+      for (auto& x : v) {
+        const int n = std::min(cmax, x.m - c0);
+        if (n <= 0) continue;
+        char* A = (char*)x.A;
+        G g;
+        g.A = (char*)x.dinv + es * size_t(kp / kLuNB) * 2 * kLuNB * kLuNB;
+        g.lda = kLuNB;
+        g.B = A + es * (size_t(kp) + size_t(c0) * x.lda);
+        g.ldb = x.lda;
+        g.Cout = (void*)g.B;
+        g.Cin = g.Cout;
+        g.ldcin = g.ldcout = x.lda;
+        g.m = kLuNB; g.n = n; g.k = kLuNB;
+        g.alpha = 1.0; g.beta = 0.0;
+        rs.push_back(g);
       }
+      gemm(pr, row_cat, t, t, t, t, false, 0, rs);
+    };
+    // A(r0 : r0+mr, c0 : c0+n) -= A(r0 : r0+mr, kc : kc+kk) A(kc : kc+kk, c0 : c0+n)
+    auto update = [&](int r0, int rmax, int c0, int cmax, int kc, int kk) {
+      std::vector<G> gs;
+      for (auto& x : v) {
+        const int mr = std::min(rmax, x.m - r0), n = std::min(cmax, x.m - c0);
+        if (mr <= 0 || n <= 0) continue;
+        char* A = (char*)x.A;
+        G g;
+        g.A = A + es * (size_t(r0) + size_t(kc) * x.lda);
+        g.lda = x.lda;
+        g.B = A + es * (size_t(kc) + size_t(c0) * x.lda);
+        g.ldb = x.lda;
+        g.Cout = A + es * (size_t(r0) + size_t(c0) * x.lda);
+        g.Cin = g.Cout;
+        g.ldcin = g.ldcout = x.lda;
+        g.m = mr; g.n = n; g.k = kk;
+        g.alpha = -1.0; g.beta = 1.0;
+        gs.push_back(g);
+      }
+      gemm(pr, upd_cat, t, t, t, upd_mt, false, 3, gs);
+    };
+    // 128-column steps (recursive LU of two 64-column panels): the trailing
+    // matrix is read and written once per 128 columns, with K = 128.  Row
+    // interchanges commute with the deferred row updates, since each panel
+    // swaps whole rows of every column outside it.
+    constexpr int kBig = 1 << 30;
+    for (int k0 = 0; k0 < mmax; k0 += 2 * kLuNB) {
+      const int k1 = k0 + kLuNB, k2 = k1 + kLuNB;
+      panel(k0);
+      if (mmax <= k1) break;
+      row_solve(k0, k1, kLuNB);                    // U12 of the right half
+      update(k1, kBig, k1, kLuNB, k0, kLuNB);       // right half, rows below
+      panel(k1);
+      if (mmax <= k2) break;
+      row_solve(k0, k2, kBig);                     // U12 rows of the left half
+      update(k1, kLuNB, k2, kBig, k0, kLuNB);       // right-half rows of the trailing columns
+      row_solve(k1, k2, kBig);                     // U12 rows of the right half
+      update(k2, kBig, k2, kBig, k0, 2 * kLuNB);    // trailing matrix, K = 128
     }
     Launch Pm;
     Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
