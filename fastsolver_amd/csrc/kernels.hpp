@@ -16,7 +16,8 @@ constexpr int kTrsmBK = 16, kTrsmBK32 = 16;   // k step of the 64-row TRSM (fp64
 constexpr int kGemmNarrowBN = 32;   // tile 2: 64 x 32 for problems at most 32 columns wide
 constexpr int kGemmNarrowBK = 32;   // k step of the narrow tiles (HBM-bound streaming products)
 constexpr int kTrsmRB = 64, kTrsmCB = 64;
-constexpr int kLuNB = 64;          // outer panel width of the blocked LU
+constexpr int kLuNB = 64;          // panel width of the blocked LU
+constexpr int kLuStep = 512;       // columns per trailing update (a power-of-two multiple of kLuNB)
 constexpr int kCopyTile = 1024;    // elements per copy workgroup
 constexpr int kSumTile = 1024;
 constexpr int64_t kNormChunk = 65536;

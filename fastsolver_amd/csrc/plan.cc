@@ -24,6 +24,7 @@
 #include <cmath>
 #include <memory>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/fastsolver_amd.h"
@@ -373,23 +374,35 @@ struct Builder {
       }
       gemm(pr, upd_cat, t, t, t, upd_mt, false, 3, gs);
     };
-    // 128-column steps (recursive LU of two 64-column panels): the trailing
-    // matrix is read and written once per 128 columns, with K = 128.  Row
-    // interchanges commute with the deferred row updates, since each panel
-    // swaps whole rows of every column outside it.
+    // Recursive LU in steps of kLuStep columns: a step factors its columns as
+    // two halves (down to 64-column panels), and the trailing matrix is read
+    // and written once per step with K = kLuStep.  Row interchanges commute
+    // with the deferred row updates, since each panel swaps whole rows of
+    // every column outside it.
     constexpr int kBig = 1 << 30;
-    for (int k0 = 0; k0 < mmax; k0 += 2 * kLuNB) {
-      const int k1 = k0 + kLuNB, k2 = k1 + kLuNB;
-      panel(k0);
-      if (mmax <= k1) break;
-      row_solve(k0, k1, kLuNB);                    // U12 of the right half
-      update(k1, kBig, k1, kLuNB, k0, kLuNB);       // right half, rows below
-      panel(k1);
+    // rows kp .. kp+h of columns c0 .. c0+cmax: L(kp : kp+h, kp : kp+h)^-1 times them
+    std::function<void(int, int, int, int)> trsm_rows = [&](int kp, int h, int c0, int cmax) {
+      if (h == kLuNB) { row_solve(kp, c0, cmax); return; }
+      trsm_rows(kp, h / 2, c0, cmax);
+      update(kp + h / 2, h / 2, c0, cmax, kp, h / 2);
+      trsm_rows(kp + h / 2, h / 2, c0, cmax);
+    };
+    // factor columns kp .. kp+w (every row below kp), earlier updates applied
+    std::function<void(int, int)> factor = [&](int kp, int w) {
+      if (w == kLuNB) { panel(kp); return; }
+      const int h = w / 2;
+      factor(kp, h);
+      if (mmax <= kp + h) return;
+      trsm_rows(kp, h, kp + h, h);                  // the right half's rows of U
+      update(kp + h, kBig, kp + h, h, kp, h);       // the right half below them
+      factor(kp + h, h);
+    };
+    for (int k0 = 0; k0 < mmax; k0 += kLuStep) {
+      factor(k0, kLuStep);
+      const int k2 = k0 + kLuStep;
       if (mmax <= k2) break;
-      row_solve(k0, k2, kBig);                     // U12 rows of the left half
-      update(k1, kLuNB, k2, kBig, k0, kLuNB);       // right-half rows of the trailing columns
-      row_solve(k1, k2, kBig);                     // U12 rows of the right half
-      update(k2, kBig, k2, kBig, k0, 2 * kLuNB);    // trailing matrix, K = 128
+      trsm_rows(k0, kLuStep, k2, kBig);
+      update(k2, kBig, k2, kBig, k0, kLuStep);     // trailing matrix, K = kLuStep
     }
     Launch Pm;
     Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
