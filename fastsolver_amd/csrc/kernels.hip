@@ -145,15 +145,17 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 
 // ========================================================================= LU
 // Panel of kLuNB columns starting at k0, factored as kLuNB/PW sub-panels of PW
-// columns held in registers: 256 threads, thread t owns rows ks + t + q*256
-// (q < RPT) of the sub-panel, RPT = 1, 2 or 4 (the smallest that covers the
-// batch's largest block).  A 256-thread group keeps the column steps' barriers
-// cheap and lets two blocks share a CU, so one block's pivot latency hides
-// behind the other's.  PW = 16 (fp64, or fp32 with RPT = 4) or 32 (fp32).
+// columns held in registers: NT = 256 threads, thread t owns rows ks + t +
+// q*NT (q < RPT) of the sub-panel, RPT = 1, 2 or 4 (the smallest that covers
+// the batch's largest block).  A 256-thread group keeps the column steps'
+// barriers cheap and lets two blocks share a CU, so one block's pivot latency
+// hides behind the other's.  Batches of many blocks of at most 256 rows use
+// one wave per block (NT = 64).  PW = 16 (fp64, or fp32 with RPT = 4) or 32 (fp32).
 // Partial pivoting picks the first largest |a| (idamax); the column below a
 // zero pivot is left unscaled and info records it, as dgetf2 does.
 constexpr int kPanelNT = 256;
 constexpr int kPanelThreads = 4 * kPanelNT;   // largest block the panel kernel holds
+constexpr int kPanelManyBlocks = 4096;        // batches at least this large use one wave per block
 
 // sub-panel width: a[RPT][PW] stays within 128 registers
 template <class T, int RPT> struct PanelWidth { static constexpr int value = sizeof(T) == 4 && RPT < 4 ? 32 : 16; };
@@ -211,9 +213,9 @@ __device__ __forceinline__ void swap_rows_in_column(PT A, int64_t col, int r1, i
   }
 }
 
-template <class T, int RPT>
-__global__ __launch_bounds__(kPanelNT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
-  constexpr int NT = kPanelNT, NW = NT / 64;
+template <class T, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+  constexpr int NW = NT / 64;
   constexpr int PW = PanelWidth<T, RPT>::value;
   constexpr int NSP = kLuNB / PW;
   constexpr int PX = kLuNB + 1;
@@ -830,15 +832,21 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (rows_max > kPanelThreads) return hipErrorInvalidValue;
-  dim3 g(n), b(kPanelNT);
-#define HODLR_PANEL(RPT)                                                                     \
-  do {                                                                                       \
-    if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, RPT>), g, b, 0, s, d, k0);    \
-    else hipLaunchKernelGGL((lu_panel_kernel<float, RPT>), g, b, 0, s, d, k0);              \
+  dim3 g(n);
+#define HODLR_PANEL(NT, RPT)                                                                         \
+  do {                                                                                               \
+    if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT, RPT>), g, dim3(NT), 0, s, d, k0); \
+    else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT>), g, dim3(NT), 0, s, d, k0);           \
   } while (0)
-  if (rows_max <= kPanelNT) HODLR_PANEL(1);
-  else if (rows_max <= 2 * kPanelNT) HODLR_PANEL(2);
-  else HODLR_PANEL(4);
+  if (n >= kPanelManyBlocks && rows_max <= 4 * 64) {
+    // many small blocks: one wave per block (no cross-wave pivot combine, and
+    // four times as many blocks in flight per CU)
+    if (rows_max <= 64) HODLR_PANEL(64, 1);
+    else if (rows_max <= 128) HODLR_PANEL(64, 2);
+    else HODLR_PANEL(64, 4);
+  } else if (rows_max <= kPanelNT) HODLR_PANEL(kPanelNT, 1);
+  else if (rows_max <= 2 * kPanelNT) HODLR_PANEL(kPanelNT, 2);
+  else HODLR_PANEL(kPanelNT, 4);
 #undef HODLR_PANEL
   return hipGetLastError();
 }

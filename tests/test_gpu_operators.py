@@ -113,7 +113,7 @@ def test_bf16_gemm_integer_known_answer_and_rounding():
         assert np.abs(dC.download() - A.astype(np.float64) @ B.astype(np.float64)).max() > 1e-4
 
 
-@pytest.mark.parametrize("n", [64, 200, 1024])
+@pytest.mark.parametrize("n", [64, 130, 200, 513, 777, 1024])
 def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     """The in-place LU factors equal LAPACK getrf's: the pivot search picks the
     same rows (first largest |a|, dgetf2's idamax), so L and U agree to rounding
@@ -127,3 +127,46 @@ def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     assert info[0] == 0
     got = dA.download()
     assert np.abs(got - lu).max() <= 1e-10 * np.abs(lu).max()
+
+
+def test_dgesv_batched_uneven_sizes_in_one_call():
+    """One batched call over blocks of different orders (the recursive LU runs
+    each step over the largest block; smaller ones drop out of the late steps):
+    every block's factors and solution match LAPACK's."""
+    import scipy.linalg
+    sizes = [33, 130, 513, 777, 1024, 64]
+    rng = np.random.default_rng(7)
+    As = [np.asfortranarray(rng.standard_normal((n, n))) for n in sizes]
+    Bs = [np.asfortranarray(rng.standard_normal((n, 3))) for n in sizes]
+    dA = [hodlr.DeviceArray.from_host(A) for A in As]
+    dB = [hodlr.DeviceArray.from_host(B) for B in Bs]
+    info = hodlr.dgesv_batched(dA, dB)
+    assert list(info) == [0] * len(sizes)
+    for n, A, B, a, b in zip(sizes, As, Bs, dA, dB):
+        lu, _ = scipy.linalg.lu_factor(A)
+        assert np.abs(a.download() - lu).max() <= 1e-10 * np.abs(lu).max(), n
+        X = b.download()
+        assert np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)) < 1e-14, n
+
+
+def test_dgesv_batched_many_small_blocks():
+    """A batch of 4096 blocks of 16..256 rows (the shape of many-system plans
+    with small leaves) takes the one-wave-per-block panel path: every block
+    solves to rounding level and sampled factors match LAPACK's."""
+    import scipy.linalg
+    rng = np.random.default_rng(11)
+    sizes = [int(x) for x in rng.integers(16, 257, 4096)]
+    As = [np.asfortranarray(rng.standard_normal((n, n))) for n in sizes]
+    Bs = [np.asfortranarray(rng.standard_normal((n, 1))) for n in sizes]
+    dA = [hodlr.DeviceArray.from_host(A) for A in As]
+    dB = [hodlr.DeviceArray.from_host(B) for B in Bs]
+    info = hodlr.dgesv_batched(dA, dB)
+    assert not np.any(info)
+    worst = 0.0
+    for A, B, b in zip(As, Bs, dB):
+        X = b.download()
+        worst = max(worst, np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)))
+    assert worst < 1e-14
+    for i in range(0, 4096, 409):
+        lu, _ = scipy.linalg.lu_factor(As[i])
+        assert np.abs(dA[i].download() - lu).max() <= 1e-10 * np.abs(lu).max(), sizes[i]
