@@ -149,13 +149,12 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
 // q*NT (q < RPT) of the sub-panel, RPT = 1, 2 or 4 (the smallest that covers
 // the batch's largest block).  A 256-thread group keeps the column steps'
 // barriers cheap and lets two blocks share a CU, so one block's pivot latency
-// hides behind the other's.  Batches of many blocks of at most 256 rows use
-// one wave per block (NT = 64).  PW = 16 (fp64, or fp32 with RPT = 4) or 32 (fp32).
+// hides behind the other's.  Panels of at most 256 rows use one wave per
+// block (NT = 64).  PW = 16 (fp64, or fp32 with RPT = 4) or 32 (fp32).
 // Partial pivoting picks the first largest |a| (idamax); the column below a
 // zero pivot is left unscaled and info records it, as dgetf2 does.
 constexpr int kPanelNT = 256;
 constexpr int kPanelThreads = 4 * kPanelNT;   // largest block the panel kernel holds
-constexpr int kPanelManyBlocks = 4096;        // batches at least this large use one wave per block
 
 // sub-panel width: a[RPT][PW] stays within 128 registers
 template <class T, int RPT> struct PanelWidth { static constexpr int value = sizeof(T) == 4 && RPT < 4 ? 32 : 16; };
@@ -838,9 +837,9 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
     if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT, RPT>), g, dim3(NT), 0, s, d, k0); \
     else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT>), g, dim3(NT), 0, s, d, k0);           \
   } while (0)
-  if (n >= kPanelManyBlocks && rows_max <= 4 * 64) {
-    // many small blocks: one wave per block (no cross-wave pivot combine, and
-    // four times as many blocks in flight per CU)
+  if (rows_max <= 4 * 64) {
+    // short panels: one wave per block (no cross-wave pivot combine, and four
+    // times as many blocks in flight per CU)
     if (rows_max <= 64) HODLR_PANEL(64, 1);
     else if (rows_max <= 128) HODLR_PANEL(64, 2);
     else HODLR_PANEL(64, 4);
