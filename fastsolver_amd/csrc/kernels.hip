@@ -132,10 +132,16 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
   const int64_t total = int64_t(d.rows) * d.cols;
   const HODLR_GLOBAL TS* src = glob((const TS*)d.src);
   HODLR_GLOBAL TD* dst = glob((TD*)d.dst);
+  // (row, column) of the tile's first element, divided once per workgroup;
+  // an element past the column's end (rows < 2^31 - kCopyTile) wraps with a
+  // 32-bit division
+  const int64_t j0 = base / d.rows;
+  const int i0 = int(base - j0 * d.rows);
   for (int e = threadIdx.x; e < kCopyTile; e += kThreads) {
-    int64_t idx = base + e;
-    if (idx >= total) break;
-    int64_t i = idx % d.rows, j = idx / d.rows;
+    if (base + e >= total) break;
+    int i = i0 + e;
+    int64_t j = j0;
+    if (i >= d.rows) { j += i / d.rows; i %= d.rows; }
     double x = d.alpha * double(src[i + j * d.lds]);
     if (d.beta != 0.0) x += d.beta * double(dst[i + j * d.ldd]);
     if (i == j) x += d.diag;
