@@ -523,7 +523,7 @@ __global__ __launch_bounds__(128) void lu_dinv_kernel(const LuDesc* __restrict__
 // diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
 // serial substitution.
 template <class TLU, class TIN, class TOUT, class UT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(TLU) == 8 ? 3 : 4))) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
   // MT: type of the diagonal-block products and of the stored factor; UT: math
   // type of the products with the solved rows (MT, or bf16 in the bf16 plans)
   typedef TLU MT;
@@ -570,37 +570,66 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof
         gemm_loop<UT, RB, CB, BKU, false>(tu, s_stage, LU, ldlu, r0, m, Xout, ldout, c0, ncols, 0, r0);
       else
         gemm_loop<UT, RB, CB, BKU, false>(tu, s_stage, LU, ldlu, r0, r0 + nr, Xout, ldout, c0, ncols, r0 + RB, m);
-      // right-hand-side rows of the step, read coalesced (64 rows of a column
-      // per wave) into s_t; then s_t -= the product with the solved rows
-      for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
-        const int r = e % RB, c = e / RB;
-        MT x = 0;
-        if (r < nr && c < nc) {
-          if (pass == 0) {
-            const int src = perm ? perm[r0 + r] : r0 + r;
-            x = MT(Xin[src + int64_t(c0 + c) * ldin]);
-          } else {
-            x = MT(Xout[(r0 + r) + int64_t(c0 + c) * ldout]);
-          }
-        }
-        s_t[r * PT + c] = x;
-      }
+      // Right-hand-side rows of the step into s_t, read coalesced (thread t
+      // owns row t % RB of columns t / RB + q * CS, 64 rows of a column per
+      // wave), and the inverted diagonal block into s_u.  Each of the two is
+      // loaded whole before its LDS writes, so the step waits for memory twice
+      // rather than once per element.
       const HODLR_GLOBAL TLU* inv = DI + int64_t(b) * 2 * RB * RB + (pass == 0 ? 0 : RB * RB);
       constexpr int KL = COMPACT ? HK : RB;   // k rows of the inverse held at once
-      for (int e = threadIdx.x; e < RB * KL; e += kThreads) {
-        const int i = e % RB, k = e / RB;
-        s_u[k * P + i] = inv[e];   // As[k][m] = inv(m = i, k)
+      constexpr int NQ = RB * CB / kThreads, CS = kThreads / RB, ND = RB * KL / kThreads;
+      static_assert(kThreads % RB == 0 && (RB * KL) % kThreads == 0, "fill shapes");
+      // (the column index t / RB is one wave's: kept in scalar registers, so
+      // each load is a scalar column base plus the thread's row)
+      const int fr = threadIdx.x % RB, fc = __builtin_amdgcn_readfirstlane(threadIdx.x / RB);
+      MT xv[NQ];
+      TLU dv[ND];
+      if (pass == 0) {
+        const int src = fr < nr ? (perm ? perm[r0 + fr] : r0 + fr) : 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int c = fc + q * CS;
+          const HODLR_GLOBAL TIN* col = Xin + int64_t(c0 + c) * ldin;
+          xv[q] = MT(*(fr < nr && c < nc ? col + src : zero_page<TIN>()));
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int c = fc + q * CS;
+          const HODLR_GLOBAL TOUT* col = Xout + int64_t(c0 + c) * ldout;
+          xv[q] = MT(*(fr < nr && c < nc ? col + (r0 + fr) : zero_page<TOUT>()));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) s_t[fr * PT + fc + q * CS] = xv[q];
+#pragma unroll
+      for (int q = 0; q < ND; ++q) dv[q] = inv[threadIdx.x + q * kThreads];
+#pragma unroll
+      for (int q = 0; q < ND; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        s_u[(e / RB) * P + e % RB] = dv[q];   // As[k][m] = inv(m = e % RB, k = e / RB)
       }
       __syncthreads();
-      tu.for_each([&](int r, int c, typename Mfma<UT>::acc_e v) { s_t[r * PT + c] -= MT(v); });
+      // s_t -= the product with the solved rows: all reads, then all writes
+      {
+        typedef typename Mfma<UT>::acc_e AE;
+        MT old[TileU::WM * TileU::WN * 4];
+        int k = 0;
+        tu.for_each([&](int r, int c, AE) { old[k++] = s_t[r * PT + c]; });
+        k = 0;
+        tu.for_each([&](int r, int c, AE v) { s_t[r * PT + c] = old[k++] - MT(v); });
+      }
       __syncthreads();
       t.zero();
       t.template mma<KL>(s_u, s_t, P, PT);
       __syncthreads();   // every wave has read s_u (and s_t)
       if constexpr (COMPACT) {
-        for (int e = threadIdx.x; e < RB * HK; e += kThreads) {
-          const int i = e % RB, k = e / RB;
-          s_u[k * P + i] = inv[e + RB * HK];   // inverse columns HK .. RB-1
+#pragma unroll
+        for (int q = 0; q < ND; ++q) dv[q] = inv[RB * HK + threadIdx.x + q * kThreads];   // columns HK .. RB-1
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+          const int e = threadIdx.x + q * kThreads;
+          s_u[(e / RB) * P + e % RB] = dv[q];
         }
         __syncthreads();
         t.template mma<HK>(s_u, s_t + HK * PT, P, PT);
@@ -608,9 +637,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof
       }
       t.for_each([&](int r, int c, typename Mfma<MT>::acc_e v) { s_t[r * PT + c] = v; });
       __syncthreads();
-      for (int e = threadIdx.x; e < RB * CB; e += kThreads) {
-        const int r = e % RB, c = e / RB;
-        if (r < nr && c < nc) Xout[(r0 + r) + int64_t(c0 + c) * ldout] = TOUT(s_t[r * PT + c]);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) xv[q] = s_t[fr * PT + fc + q * CS];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = fc + q * CS;
+        if (fr < nr && c < nc) (Xout + int64_t(c0 + c) * ldout)[r0 + fr] = TOUT(xv[q]);
       }
       __syncthreads();
     }
