@@ -343,9 +343,19 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
     }
     __syncthreads();
     if (nright > 0) {
-      for (int e = tid; e < PW * nright; e += NT) {
-        const int r = e % PW, c = e / PW;
-        s_x[r * PX + c] = A[(ks + r) + int64_t(ks + PW + c) * lda];
+      // 4 loads in flight per thread before their LDS writes
+      for (int e0 = tid; e0 < PW * nright; e0 += 4 * NT) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * NT;
+          v[u] = *(e < PW * nright ? A + (ks + e % PW) + int64_t(ks + PW + e / PW) * lda : zero_page<T>());
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * NT;
+          if (e < PW * nright) s_x[(e % PW) * PX + e / PW] = v[u];
+        }
       }
       __syncthreads();
       // U12 = L11^-1 A12 (unit lower), one thread per column
@@ -476,9 +486,25 @@ __global__ __launch_bounds__(128) void lu_dinv_kernel(const LuDesc* __restrict__
   const HODLR_GLOBAL T* A = glob((const T*)d.A);
   // s_a[k*kDB + i] = A(r0+i, r0+k); the identity outside the nr x nr corner
   __shared__ __attribute__((aligned(16))) T s_a[kDB * kDB];
-  for (int e = threadIdx.x; e < kDB * kDB; e += 128) {
-    const int i = e % kDB, k = e / kDB;
-    s_a[e] = (i < nr && k < nr) ? A[(r0 + i) + int64_t(k + r0) * d.lda] : (i == k ? T(1) : T(0));
+  {
+    // thread t reads row t % kDB of columns t / kDB + 2q, 8 columns' loads in
+    // flight before their LDS writes
+    const int i = threadIdx.x % kDB, w = __builtin_amdgcn_readfirstlane(threadIdx.x / kDB);
+#pragma unroll
+    for (int q0 = 0; q0 < kDB / 2; q0 += 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = w + 2 * (q0 + u);
+        const HODLR_GLOBAL T* col = A + int64_t(k + r0) * d.lda;
+        v[u] = *(i < nr && k < nr ? col + (r0 + i) : zero_page<T>());
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = w + 2 * (q0 + u);
+        s_a[k * kDB + i] = (i < nr && k < nr) ? v[u] : (i == k ? T(1) : T(0));
+      }
+    }
   }
   __syncthreads();
   const int j = threadIdx.x & 63;
