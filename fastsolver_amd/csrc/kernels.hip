@@ -137,15 +137,33 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
   // 32-bit division
   const int64_t j0 = base / d.rows;
   const int i0 = int(base - j0 * d.rows);
-  for (int e = threadIdx.x; e < kCopyTile; e += kThreads) {
-    if (base + e >= total) break;
+  // all of a thread's loads are issued before its stores (past the end: the
+  // zero page, not stored)
+  constexpr int NE = kCopyTile / kThreads;
+  const bool useb = d.beta != 0.0;
+  int ii[NE];
+  int64_t jj[NE];
+  TS v[NE];
+  TD w[NE];
+#pragma unroll
+  for (int u = 0; u < NE; ++u) {
+    const int e = threadIdx.x + u * kThreads;
     int i = i0 + e;
     int64_t j = j0;
     if (i >= d.rows) { j += i / d.rows; i %= d.rows; }
-    double x = d.alpha * double(src[i + j * d.lds]);
-    if (d.beta != 0.0) x += d.beta * double(dst[i + j * d.ldd]);
-    if (i == j) x += d.diag;
-    dst[i + j * d.ldd] = TD(x);
+    const bool ok = base + e < total;
+    ii[u] = i;
+    jj[u] = j;
+    v[u] = *(ok ? src + i + j * d.lds : zero_page<TS>());
+    w[u] = useb ? *(ok ? dst + i + j * d.ldd : zero_page<TD>()) : TD(0);
+  }
+#pragma unroll
+  for (int u = 0; u < NE; ++u) {
+    if (base + threadIdx.x + u * kThreads >= total) break;
+    double x = d.alpha * double(v[u]);
+    if (useb) x += d.beta * double(w[u]);
+    if (ii[u] == jj[u]) x += d.diag;
+    dst[ii[u] + jj[u] * d.ldd] = TD(x);
   }
 }
 
