@@ -5,7 +5,7 @@
 //                    Serves the level reduce V^T [d|u] (src/solver/gemm.cc:315-384),
 //                    the broadcast d -= u*eta (gemm.cc:417-478), the node-solve
 //                    products (src/solver/solver_tasks.cc:236-302), the LU
-//                    trailing updates and the HODLR matvec.
+//                    row solves and trailing updates and the HODLR matvec.
 //   lu_*_kernel      batched blocked LU with partial pivoting (getrf semantics)
 //                    of the leaf blocks (solver_tasks.cc:437) and of the r x r
 //                    Schur complements (solver_tasks.cc:281).
@@ -487,7 +487,7 @@ __global__ void lu_init_kernel(const LuDesc* __restrict__ descs, int n) {
 // For the 64-row diagonal block b0 + blockIdx.x of every LU factor: L_bb^-1
 // (unit lower) and U_bb^-1 (upper), zero outside the block's nr x nr corner.
 // The block is final once its panel is factored, so the plan inverts it right
-// after the panel: the row-block solve U12 = L11^-1 A12 and the leaf TRSM
+// after the panel: the solve for the U rows, U12 = L11^-1 A12, and the leaf TRSM
 // then run as MFMA products.  Wave 0 builds column j = lane of L^-1, wave 1
 // column j of U^-1, each column held in registers and updated column-oriented
 // (x_i -= A_ik x_k for all i at once), so the 64-step chains carry up to 63

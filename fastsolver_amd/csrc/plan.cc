@@ -6,8 +6,9 @@
 // same height, across all systems of the plan, is one batched kernel launch
 // over a descriptor array built once at plan creation:
 //
-//   leaf      K copy -> LU (blocked, 64-wide panels: panel / row-block / update) -> perm
-//             -> TRSM of [b | U] (solver_tasks.cc:424-438)
+//   leaf      K copy -> LU (recursive, 512-column steps of 64-wide panels: panel /
+//             inverse of its diagonal block / U rows as MFMA products / trailing
+//             update) -> perm -> TRSM of [b | U] (solver_tasks.cc:424-438)
 //   height h  reduce   W_i = V_i^T X_i[:, 0:c+R_l+r]  (split-K + fixed-order sum)
 //             node     VTd1 -= VTu1 VTd0;  S = I - VTu1 VTu0;  LU(S);  eta0 = S^-1 VTd1;
 //                      eta1 = VTd0 - VTu0 eta0          (solver_tasks.cc:216-302)
@@ -22,9 +23,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <memory>
 #include <string>
-#include <functional>
 #include <vector>
 
 #include "../../include/fastsolver_amd.h"
