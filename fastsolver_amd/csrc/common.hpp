@@ -199,7 +199,9 @@ struct Tile<bf16, BM, BN, BK> {
 // the storage type: conversion to the math type happens at the LDS store,
 // after the MFMA work of the current stage, so no wait sits between a load and
 // the MFMAs that run while it is in flight.
-template <class MT, class TA, class TB, int BM, int BN, int BK, bool TRANS>
+// NTB: B loads carry the nontemporal hint (streamed operands that should not
+// displace the reused A operand from L2)
+template <class MT, class TA, class TB, int BM, int BN, int BK, bool TRANS, bool NTB = false>
 struct Stager {
   static constexpr int NA = (BM * BK) / kThreads, NB = (BN * BK) / kThreads;
   static constexpr int PA = BM + 1, PB = BN + 1;
@@ -237,7 +239,11 @@ struct Stager {
     const uint32_t voff = uint32_t(tid % INNER) + uint32_t(tid / INNER) * uint32_t(ldb);
     const HODLR_GLOBAL TB* base = B + k0 + int64_t(c0) * ldb;
 #pragma unroll
-    for (int s = 0; s < NB; ++s) rb[s] = (base + int64_t(s) * STEP * ldb)[voff];
+    for (int s = 0; s < NB; ++s) {
+      const HODLR_GLOBAL TB* p = &(base + int64_t(s) * STEP * ldb)[voff];
+      if constexpr (NTB) rb[s] = __builtin_nontemporal_load(p);
+      else rb[s] = *p;
+    }
   }
   __device__ __forceinline__ void load(const HODLR_GLOBAL TA* A, int64_t lda, int r0, int mlim,
                                        const HODLR_GLOBAL TB* B, int64_t ldb, int c0, int nlim, int k0, int klim) {
@@ -253,7 +259,9 @@ struct Stager {
       const int k = e % BK, n = e / BK;
       const int gk = k0 + k, gn = c0 + n;
       const bool ok = gk < klim && gn < nlim;
-      rb[s] = *(ok ? B + gk + int64_t(gn) * ldb : zero_page<TB>());
+      const HODLR_GLOBAL TB* p = ok ? B + gk + int64_t(gn) * ldb : zero_page<TB>();
+      if constexpr (NTB) rb[s] = __builtin_nontemporal_load(p);
+      else rb[s] = *p;
     }
   }
   __device__ __forceinline__ void store(MT* As, MT* Bs) const {
@@ -272,9 +280,9 @@ struct Stager {
 };
 
 // bf16 staging: the same loads, stored octet-planar (see Tile<bf16>)
-template <class TA, class TB, int BM, int BN, int BK, bool TRANS>
-struct Stager<bf16, TA, TB, BM, BN, BK, TRANS> : Stager<float, TA, TB, BM, BN, BK, TRANS> {
-  typedef Stager<float, TA, TB, BM, BN, BK, TRANS> Base;
+template <class TA, class TB, int BM, int BN, int BK, bool TRANS, bool NTB>
+struct Stager<bf16, TA, TB, BM, BN, BK, TRANS, NTB> : Stager<float, TA, TB, BM, BN, BK, TRANS, NTB> {
+  typedef Stager<float, TA, TB, BM, BN, BK, TRANS, NTB> Base;
   using Base::NA;
   using Base::NB;
   __device__ __forceinline__ void store(bf16* As, bf16* Bs) const {
@@ -295,13 +303,13 @@ struct Stager<bf16, TA, TB, BM, BN, BK, TRANS> : Stager<float, TA, TB, BM, BN, B
 
 // acc += op(A)[r0.., kb..ke) * B[kb..ke), c0..) with register prefetch and two
 // LDS stages: global loads of step t+1 are in flight while step t computes.
-template <class MT, int BM, int BN, int BK, bool TRANS, class TA, class TB>
+template <class MT, int BM, int BN, int BK, bool TRANS, bool NTB = false, class TA, class TB>
 __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, const HODLR_GLOBAL TA* A, int64_t lda,
                                           int r0, int mlim, const HODLR_GLOBAL TB* B, int64_t ldb, int c0, int nlim,
                                           int kb, int ke) {
   typedef Tile<MT, BM, BN, BK> T;
   if (kb >= ke) return;
-  Stager<MT, TA, TB, BM, BN, BK, TRANS> st;
+  Stager<MT, TA, TB, BM, BN, BK, TRANS, NTB> st;
   st.load(A, lda, r0, mlim, B, ldb, c0, nlim, kb, ke);
   st.store(lds, lds + T::BOFF);
   __syncthreads();
