@@ -89,6 +89,7 @@ GEN_SYMBOLS = {
     "hodlr_gen_lapack_name": ([ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "hodlr_gen_circulant": ([_i64, _i64, ctypes.c_int, ctypes.c_int, _dbl, _i64, _i64, _ptr, _i64, _ptr, _i64, _ptr,
                              _i64, ctypes.c_int], ctypes.c_int),
+    "hodlr_gen_rhs_drand48": ([_i64, _i64, ctypes.c_int, ctypes.c_int, _i64, _i64, _ptr, _i64], ctypes.c_int),
     "hodlr_gen_random": ([_i64, _i64, ctypes.c_int, ctypes.c_int, _ptr, _ptr, _ptr, _i64, _ptr, _i64, _ptr, _i64,
                           ctypes.c_int], ctypes.c_int),
     "hodlr_gen_tree": ([_i64, _i64, ctypes.c_int, ctypes.c_int, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),

@@ -171,15 +171,16 @@ int hodlr_gen_lapack_name(char* buf, int len) {
   return 0;
 }
 
-int hodlr_gen_circulant(int64_t n, int64_t threshold, int nrhs, int rank, double diag, int64_t seed,
-                        int64_t rhs_region_rows, double* X, int64_t ldx, double* V, int64_t ldv, double* K,
-                        int64_t ldk, int klayout) {
-  hodlr::Tree t;
-  int rc = hodlr::build_tree(n, threshold, nrhs, 1, &rank, t);
-  if (rc) return rc;
-  const int64_t r = rank;
-  // right-hand side: drand48 reseeded per region, row-major draws
-  // (init_matrix_tasks.cc:67-76; regions are the leaves, hodlr_matrix.cc:191-209)
+}  // extern "C"
+
+namespace {
+// Right-hand side of the reference: drand48 reseeded with `seed` at the start
+// of every region, values drawn row by row into the column-major block
+// (init_matrix_tasks.cc:67-76); the regions are the leaves
+// (hodlr_matrix.cc:191-209) or blocks of rhs_region_rows rows.
+void rhs_drand48(const hodlr::Tree& t, int64_t seed, int64_t rhs_region_rows, double* X, int64_t ldx) {
+  const int64_t n = t.n;
+  const int nrhs = t.nrhs;
   std::vector<std::pair<int64_t, int64_t>> regions;
   if (rhs_region_rows > 0) {
     for (int64_t b = 0; b < n; b += rhs_region_rows) regions.push_back({b, std::min(rhs_region_rows, n - b)});
@@ -195,6 +196,28 @@ int hodlr_gen_circulant(int64_t n, int64_t threshold, int nrhs, int rank, double
         X[regions[g].first + i + j * ldx] = ldexp(double(x), -48);
       }
   }
+}
+}  // namespace
+
+extern "C" {
+
+int hodlr_gen_rhs_drand48(int64_t n, int64_t threshold, int nrhs, int rank, int64_t seed,
+                          int64_t rhs_region_rows, double* X, int64_t ldx) {
+  hodlr::Tree t;
+  int rc = hodlr::build_tree(n, threshold, nrhs, 1, &rank, t);
+  if (rc) return rc;
+  rhs_drand48(t, seed, rhs_region_rows, X, ldx);
+  return 0;
+}
+
+int hodlr_gen_circulant(int64_t n, int64_t threshold, int nrhs, int rank, double diag, int64_t seed,
+                        int64_t rhs_region_rows, double* X, int64_t ldx, double* V, int64_t ldv, double* K,
+                        int64_t ldk, int klayout) {
+  hodlr::Tree t;
+  int rc = hodlr::build_tree(n, threshold, nrhs, 1, &rank, t);
+  if (rc) return rc;
+  const int64_t r = rank;
+  rhs_drand48(t, seed, rhs_region_rows, X, ldx);
   // U and V: value (i_global + j) mod r in every level block
   // (init_matrix_tasks.cc:208-218; hodlr_matrix.cc:476-497)
 #pragma omp parallel for schedule(static)

@@ -91,6 +91,7 @@ struct SysState {
   int64_t ldw = 0;                 // leading dimension of the workspaces (= N)
   size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0, ldinv = 0;
   size_t vws = 0;                  // FP32 plans: V rounded to FP32 once per factor pass (N x R, ld N)
+  size_t mvx = 0;                  // hodlr_matvec: the caller's x staged here (N x c FP64, ld N)
   std::vector<size_t> lu_off;      // per leaf (elements)
   std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
   std::vector<size_t> z_off;       // per leaf (doubles)
@@ -126,6 +127,8 @@ struct hodlr_plan_s {
   int32_t* node_info = nullptr;
   // procedures
   Proc leaf, extract, residual, update;
+  Proc matvec;       // rbuf = A mvx (hodlr_matvec)
+  Proc solve_out;    // rbuf = yws (FP64), the result of a correction solve
   std::vector<Proc> reduce, node, bcast;          // index = height
   std::vector<Proc> reduce_s, node_s, bcast_s;    // solve-only (c columns)
   Proc solve_only_leaf;
@@ -141,6 +144,11 @@ struct hodlr_plan_s {
   double prof_flops[CAT_N] = {0}, prof_bytes[CAT_N] = {0};
   struct Pending { int cat; hipEvent_t a, b; double flops, bytes; };
   std::vector<Pending> pending;
+  // event pairs recorded as nodes of the captured graph: every replay records
+  // them again, so they are read after each replay, before the next one
+  bool capturing = false;
+  std::vector<Pending> graph_ev;
+  bool graph_ev_armed = false;
   std::vector<hipEvent_t> pool;
   // GEMM tilings: per tunable launch the chosen tile (0: 64 x 64, 1: 128 x 128)
   std::vector<int> tile_choice;
@@ -487,6 +495,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     S.xws = L.take(size_t(n) * t.xcols() * we);
     S.yws = L.take(size_t(n) * c * we);
     S.rbuf = L.take(size_t(n) * c * 8);
+    S.mvx = L.take(size_t(n) * c * 8);
     if (we != 8) S.vws = L.take(size_t(n) * std::max<int64_t>(t.vcols(), 1) * we);
     S.lpiv = L.take(size_t(n) * 4);
     S.lperm = L.take(size_t(n) * 4);
@@ -587,6 +596,66 @@ void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, s
     sd.m = r; sd.n = int(c); sd.nsplit = S.leaf_hi[i] - S.leaf_lo[i];
     sums.push_back(sd);
   }
+}
+
+// HODLR product into the plan's rbuf, for every system:
+//   from_solution: rbuf = b + sign * A x with x the caller's solution array and
+//                  b = X[:, 0:c] (sign = -1: the residual b - A x);
+//   otherwise:     rbuf = sign * A mvx (the staged input of hodlr_matvec).
+// A x = blockdiag(K) x + sum over nodes of U_child (V_sibling^T x_sibling):
+// the diagonal-block products, then every leaf's U rows times the gathered
+// V^T x blocks of its ancestors' siblings (Z_j), all in FP64.
+void hodlr_product(hodlr_plan_s* P, Builder& B, Proc& pr, bool from_solution, double sign) {
+  std::vector<Builder::G> red, lk, lu;
+  std::vector<SumDesc> wsum;
+  std::vector<CopyDesc> gz;
+  for (SysState& S : P->sys) {
+    const Tree& t = S.tree;
+    const int64_t c = t.nrhs;
+    double* R = at<double>(P, S.rbuf);
+    double* Z = at<double>(P, S.zbuf);
+    const double* x = from_solution ? S.s.x : at<double>(P, S.mvx);
+    const int64_t ldx = from_solution ? S.s.ldxs : S.ldw;
+    matvec_reduce(P, S, x, ldx, red, wsum);
+    for (size_t j = 0; j < t.leaves.size(); ++j) {
+      const int id = t.leaves[j];
+      const TreeNode& nd = t.nodes[id];
+      Builder::G g{};
+      g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : nd.nrow;
+      g.B = x + nd.row_beg; g.ldb = ldx;
+      g.Cin = from_solution ? S.s.X + nd.row_beg : nullptr; g.ldcin = S.s.ldx;
+      g.Cout = R + nd.row_beg; g.ldcout = S.ldw;
+      g.m = int(nd.nrow); g.n = int(c); g.k = int(nd.nrow); g.alpha = sign; g.beta = from_solution ? 1.0 : 0.0;
+      lk.push_back(g);
+      const int64_t Rd = t.rsum[nd.depth];
+      if (Rd == 0) continue;
+      // Z_j rows of depth l: w of the sibling of the depth-l ancestor
+      int a = id;
+      while (a != 0) {
+        const TreeNode& an = t.nodes[a];
+        const TreeNode& par = t.nodes[an.parent];
+        const int sib = par.child[0] == a ? par.child[1] : par.child[0];
+        const int r = t.rank(an.depth);
+        CopyDesc cd{};
+        cd.src = at<double>(P, S.wmv[sib]); cd.lds = r;
+        cd.dst = Z + S.z_off[j] + t.rsum[an.depth - 1]; cd.ldd = Rd;
+        cd.rows = r; cd.cols = int(c); cd.alpha = 1.0;
+        gz.push_back(cd);
+        a = an.parent;
+      }
+      Builder::G u{};
+      u.A = S.s.X + nd.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
+      u.B = Z + S.z_off[j]; u.ldb = Rd;
+      u.Cin = R + nd.row_beg; u.ldcin = S.ldw; u.Cout = R + nd.row_beg; u.ldcout = S.ldw;
+      u.m = int(nd.nrow); u.n = int(c); u.k = int(Rd); u.alpha = sign; u.beta = 1.0;
+      lu.push_back(u);
+    }
+  }
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
+  B.sum(pr, CAT_MATVEC, wsum);
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
+  B.copy(pr, CAT_MATVEC, F64, F64, gz);
+  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
 }
 
 void build_all(hodlr_plan_s* P, Builder& B) {
@@ -794,63 +863,36 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     B.copy(P->update, CAT_COPY, wt, F64, up);
   }
 
-  // ---------------------------------------------------------------- residual r = b - A x
+  // ---------------------------------------------------------------- residual r = b - A x, ||b||, ||r||
   {
-    std::vector<Builder::G> red, lk, lu;
-    std::vector<SumDesc> wsum;
-    std::vector<CopyDesc> gz;
     std::vector<NormDesc> nr;
     for (SysState& S : P->sys) {
-      const Tree& t = S.tree;
-      const int64_t c = t.nrhs;
-      double* R = at<double>(P, S.rbuf);
-      double* Z = at<double>(P, S.zbuf);
-      matvec_reduce(P, S, S.s.x, S.s.ldxs, red, wsum);
-      for (size_t j = 0; j < t.leaves.size(); ++j) {
-        const int id = t.leaves[j];
-        const TreeNode& x = t.nodes[id];
-        Builder::G g{};
-        g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : x.nrow;
-        g.B = S.s.x + x.row_beg; g.ldb = S.s.ldxs;
-        g.Cin = S.s.X + x.row_beg; g.ldcin = S.s.ldx;
-        g.Cout = R + x.row_beg; g.ldcout = S.ldw;
-        g.m = int(x.nrow); g.n = int(c); g.k = int(x.nrow); g.alpha = -1.0; g.beta = 1.0;
-        lk.push_back(g);
-        const int64_t Rd = t.rsum[x.depth];
-        if (Rd == 0) continue;
-        // Z_j rows of depth l: w of the sibling of the depth-l ancestor
-        int a = id;
-        while (a != 0) {
-          const TreeNode& an = t.nodes[a];
-          const TreeNode& par = t.nodes[an.parent];
-          const int sib = par.child[0] == a ? par.child[1] : par.child[0];
-          const int r = t.rank(an.depth);
-          CopyDesc cd{};
-          cd.src = at<double>(P, S.wmv[sib]); cd.lds = r;
-          cd.dst = Z + S.z_off[j] + t.rsum[an.depth - 1]; cd.ldd = Rd;
-          cd.rows = r; cd.cols = int(c); cd.alpha = 1.0;
-          gz.push_back(cd);
-          a = an.parent;
-        }
-        Builder::G u{};
-        u.A = S.s.X + x.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
-        u.B = Z + S.z_off[j]; u.ldb = Rd;
-        u.Cin = R + x.row_beg; u.ldcin = S.ldw; u.Cout = R + x.row_beg; u.ldcout = S.ldw;
-        u.m = int(x.nrow); u.n = int(c); u.k = int(Rd); u.alpha = -1.0; u.beta = 1.0;
-        lu.push_back(u);
-      }
+      const int64_t c = S.tree.nrhs;
       for (int64_t j = 0; j < c; ++j) {
         NormDesc d{};
-        d.x = R + j * S.ldw; d.n = t.n; d.out = at<double>(P, S.rnorm) + j;
+        d.x = at<double>(P, S.rbuf) + j * S.ldw; d.n = S.tree.n; d.out = at<double>(P, S.rnorm) + j;
         nr.push_back(d);
+        // ||b_j|| is taken at every residual: the caller may change b between solves
+        NormDesc e{};
+        e.x = S.s.X + j * S.s.ldx; e.n = S.tree.n; e.out = at<double>(P, S.bnorm) + j;
+        nr.push_back(e);
       }
     }
-    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
-    B.sum(P->residual, CAT_MATVEC, wsum);
-    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
-    B.copy(P->residual, CAT_MATVEC, F64, F64, gz);
-    B.gemm(P->residual, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
+    hodlr_product(P, B, P->residual, /*from_solution=*/true, -1.0);
     B.norms(P->residual, nr);
+  }
+  // ---------------------------------------------------------------- hodlr_matvec: rbuf = A mvx
+  hodlr_product(P, B, P->matvec, /*from_solution=*/false, 1.0);
+  // ---------------------------------------------------------------- correction solve result: rbuf = yws
+  {
+    std::vector<CopyDesc> yo;
+    for (SysState& S : P->sys) {
+      CopyDesc c{};
+      c.src = P->ws.p + S.yws; c.lds = S.ldw; c.dst = at<double>(P, S.rbuf); c.ldd = S.ldw;
+      c.rows = int(S.tree.n); c.cols = S.tree.nrhs; c.alpha = 1.0;
+      yo.push_back(c);
+    }
+    B.copy(P->solve_out, CAT_COPY, wt, F64, yo);
   }
 }
 
@@ -901,20 +943,41 @@ hipEvent_t take_event(hodlr_plan_s* P) {
 }
 
 int harvest(hodlr_plan_s* P) {
-  if (P->pending.empty()) return HODLR_OK;
+  if (P->pending.empty() && !P->graph_ev_armed) return HODLR_OK;
   if (hipStreamSynchronize(P->stream) != hipSuccess) return HODLR_ERR_HIP;
-  for (auto& q : P->pending) {
+  auto take = [&](const hodlr_plan_s::Pending& q) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) return HODLR_ERR_HIP;
+    if (hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) return false;
     P->prof_ms[q.cat] += ms;
     P->prof_n[q.cat] += 1;
     P->prof_flops[q.cat] += q.flops;
     P->prof_bytes[q.cat] += q.bytes;
+    return true;
+  };
+  for (auto& q : P->pending) {
+    if (!take(q)) return HODLR_ERR_HIP;
     P->pool.push_back(q.a);
     P->pool.push_back(q.b);
   }
   P->pending.clear();
+  if (P->graph_ev_armed) {
+    P->graph_ev_armed = false;
+    for (auto& q : P->graph_ev)
+      if (!take(q)) return HODLR_ERR_HIP;
+  }
   return HODLR_OK;
+}
+
+// Drops the captured graph (and the event pairs recorded in it).
+void drop_graph(hodlr_plan_s* P) {
+  (void)harvest(P);
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  P->graph_exec = nullptr;
+  for (auto& q : P->graph_ev) {
+    P->pool.push_back(q.a);
+    P->pool.push_back(q.b);
+  }
+  P->graph_ev.clear();
 }
 
 int run(hodlr_plan_s* P, const Proc& pr) {
@@ -934,7 +997,7 @@ int run(hodlr_plan_s* P, const Proc& pr) {
       if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
       if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
       if (hipEventRecord(b, P->stream) != hipSuccess) return HODLR_ERR_HIP;
-      P->pending.push_back({int(L.cat), a, b, L.flops, L.bytes});
+      (P->capturing ? P->graph_ev : P->pending).push_back({int(L.cat), a, b, L.flops, L.bytes});
     } else if (dispatch(P, L) != hipSuccess) {
       return HODLR_ERR_HIP;
     }
@@ -1010,10 +1073,9 @@ int check_system(const hodlr_system_t& s) {
 }
 
 void free_plan(hodlr_plan_s* P) {
-  harvest(P);
+  drop_graph(P);
   for (hipEvent_t e : P->pool) (void)hipEventDestroy(e);
   P->pool.clear();
-  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
   if (P->ws.p) (void)hipFree(P->ws.p);
   if (P->descs.p) (void)hipFree(P->descs.p);
   if (P->scratch.p) (void)hipFree(P->scratch.p);
@@ -1024,20 +1086,26 @@ int sync_stream(hodlr_plan_s* P) {
 }
 
 int run_factor(hodlr_plan_s* P) {
-  if (P->use_graph && !P->prof_mask) {
+  if (P->use_graph) {
+    // the events of the previous replay are read before they are recorded again
+    if (P->graph_ev_armed && harvest(P)) return HODLR_ERR_HIP;
     if (!P->graph_exec) {
       hipGraph_t g;
       if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
+      P->capturing = true;
       int rc = run(P, P->factor);
+      P->capturing = false;
       hipError_t e = hipStreamEndCapture(P->stream, &g);
-      if (rc || e != hipSuccess) return HODLR_ERR_HIP;
-      if (hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
-        hipGraphDestroy(g);
+      if (e != hipSuccess) return HODLR_ERR_HIP;
+      if (rc || hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        P->graph_exec = nullptr;
         return HODLR_ERR_HIP;
       }
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
     }
     if (hipGraphLaunch(P->graph_exec, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+    P->graph_ev_armed = !P->graph_ev.empty();
   } else {
     int rc = run(P, P->factor);
     if (rc) return rc;
@@ -1144,6 +1212,7 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
   Builder B{P.get()};
   B.tunable = true;
   P->leaf = Proc(); P->extract = Proc(); P->residual = Proc(); P->update = Proc(); P->solve_only_leaf = Proc();
+  P->matvec = Proc(); P->solve_out = Proc();
   build_all(P.get(), B);
   P->tile_choice.assign(B.ngid, -1);
   // the fused pass
@@ -1163,39 +1232,13 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
     return HODLR_ERR_HIP;
   }
   fill_info(P.get());
-  // norms of b (used by every residual)
-  {
-    Builder nb{P.get()};
-    Proc pr;
-    std::vector<NormDesc> v;
-    for (SysState& S : P->sys)
-      for (int j = 0; j < S.tree.nrhs; ++j) {
-        NormDesc d{};
-        d.x = S.s.X + int64_t(j) * S.s.ldx;
-        d.n = S.tree.n;
-        d.out = at<double>(P.get(), S.bnorm) + j;
-        v.push_back(d);
-      }
-    nb.norms(pr, v);
-    Buf tmp;
-    tmp.bytes = nb.blob.data.size();
-    if (hipMalloc((void**)&tmp.p, tmp.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
-    hipMemcpy(tmp.p, nb.blob.data.data(), tmp.bytes, hipMemcpyHostToDevice);
-    char* save = P->descs.p;
-    P->descs.p = tmp.p;
-    int rc = run(P.get(), pr);
-    P->descs.p = save;
-    hipError_t e = hipStreamSynchronize(P->stream);
-    hipFree(tmp.p);
-    if (rc || e != hipSuccess) { free_plan(P.get()); return HODLR_ERR_HIP; }
-  }
   *plan = P.release();
   return HODLR_OK;
 }
 
 int hodlr_plan_destroy(hodlr_plan_t P) {
   if (!P) return HODLR_ERR_ARG;
-  hipStreamSynchronize(P->stream);
+  (void)hipStreamSynchronize(P->stream);
   free_plan(P);
   delete P;
   return HODLR_OK;
@@ -1261,30 +1304,13 @@ int hodlr_solve_only(hodlr_plan_t P, const double* const* rhs, const int64_t* ld
                      const int64_t* ldo) {
   if (!P || !rhs || !ldr || !out || !ldo) return HODLR_ERR_ARG;
   if (!P->factored) return HODLR_ERR_ARG;
+  for (size_t s = 0; s < P->sys.size(); ++s)
+    if (!rhs[s] || !out[s] || ldr[s] < P->sys[s].tree.n || ldo[s] < P->sys[s].tree.n) return HODLR_ERR_ARG;
   int rc = stage_copy(P, rhs, ldr, true, nullptr, nullptr);
   if (!rc) rc = solve_only_internal(P);
-  if (rc) return rc;
-  // out = Y[:, 0:c] (converted to double)
-  for (size_t s = 0; s < P->sys.size(); ++s) {
-    SysState& S = P->sys[s];
-    Builder b{P};
-    Proc pr;
-    CopyDesc c{};
-    c.src = P->ws.p + S.yws; c.lds = S.ldw; c.dst = out[s]; c.ldd = ldo[s];
-    c.rows = int(S.tree.n); c.cols = S.tree.nrhs; c.alpha = 1.0;
-    b.copy(pr, CAT_COPY, P->wt, F64, {c});
-    void* tmp = nullptr;
-    if (hipMalloc(&tmp, b.blob.data.size()) != hipSuccess) return HODLR_ERR_NOMEM;
-    hipMemcpyAsync(tmp, b.blob.data.data(), b.blob.data.size(), hipMemcpyHostToDevice, P->stream);
-    char* save = P->descs.p;
-    P->descs.p = (char*)tmp;
-    rc = run(P, pr);
-    P->descs.p = save;
-    hipStreamSynchronize(P->stream);
-    hipFree(tmp);
-    if (rc) return rc;
-  }
-  return HODLR_OK;
+  if (!rc) rc = run(P, P->solve_out);              // rbuf <- yws (FP64)
+  if (!rc) rc = stage_copy(P, nullptr, nullptr, false, out, ldo);
+  return rc;
 }
 
 // ------------------------------------------------- standalone batched operators
@@ -1362,55 +1388,83 @@ int hodlr_sgemm_batched_bf16(int32_t batch, const int32_t* m, const int32_t* n, 
                       (const void* const*)B, ldb, beta, (void* const*)C, ldc, stream);
 }
 
-int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
-                        double* const* B, const int64_t* ldb, int32_t* info, void* stream) {
+// hodlr_dgesv_batched's launches on a workspace at `ws` (dgesv_layout's
+// offsets): LU with partial pivoting of every A_i in place, B_i copied aside,
+// then B_i = U^-1 L^-1 P (copy).
+struct DgesvLayout {
+  std::vector<size_t> piv, prm, dv, bc;
+  size_t inf = 0, bytes = 0;
+};
+
+static int dgesv_check(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
+                       double* const* B, const int64_t* ldb) {
   if (batch <= 0 || !n || !nrhs || !A || !lda || !B || !ldb) return HODLR_ERR_ARG;
   for (int i = 0; i < batch; ++i) {
-    if (n[i] <= 0 || nrhs[i] < 0 || lda[i] < n[i] || ldb[i] < n[i]) return HODLR_ERR_ARG;
+    if (n[i] <= 0 || nrhs[i] < 0 || lda[i] < n[i] || ldb[i] < n[i] || !A[i] || !B[i]) return HODLR_ERR_ARG;
     if (n[i] > kMaxBlock) return HODLR_ERR_UNSUPPORTED;
   }
-  hodlr_plan_s P;
-  P.stream = (hipStream_t)stream;
-  // workspace: pivots, permutations, info, inverted diagonal blocks, a copy of B
+  return HODLR_OK;
+}
+
+static DgesvLayout dgesv_layout(int32_t batch, const int32_t* n, const int32_t* nrhs) {
+  // pivots, permutations, inverted diagonal blocks, a copy of B, info
+  DgesvLayout D;
   Layout L;
-  std::vector<size_t> piv(batch), prm(batch), dv(batch), bc(batch);
+  D.piv.resize(batch); D.prm.resize(batch); D.dv.resize(batch); D.bc.resize(batch);
   for (int i = 0; i < batch; ++i) {
-    piv[i] = L.take(size_t(n[i]) * 4);
-    prm[i] = L.take(size_t(n[i]) * 4);
-    dv[i] = L.take(size_t(cdiv(n[i], 64)) * 2 * 64 * 64 * 8);
-    bc[i] = L.take(size_t(n[i]) * std::max(nrhs[i], 1) * 8);
+    D.piv[i] = L.take(size_t(n[i]) * 4);
+    D.prm[i] = L.take(size_t(n[i]) * 4);
+    D.dv[i] = L.take(size_t(cdiv(n[i], 64)) * 2 * 64 * 64 * 8);
+    D.bc[i] = L.take(size_t(n[i]) * std::max(nrhs[i], 1) * 8);
   }
-  size_t inf = L.take(size_t(batch) * 4);
-  if (hipMalloc((void**)&P.ws.p, L.top) != hipSuccess) return HODLR_ERR_NOMEM;
-  P.ws.bytes = L.top;
-  Builder Bd{&P};
-  Proc pr;
+  D.inf = L.take(size_t(batch) * 4);
+  D.bytes = L.top;
+  return D;
+}
+
+static void dgesv_build(hodlr_plan_s& P, Builder& Bd, Proc& pr, const DgesvLayout& D, int32_t batch,
+                        const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
+                        double* const* B, const int64_t* ldb) {
   std::vector<LuDesc> lus;
   std::vector<CopyDesc> cps;
   std::vector<TrsmDesc> trs;
   for (int i = 0; i < batch; ++i) {
     LuDesc l{};
     l.A = A[i]; l.lda = lda[i]; l.m = n[i];
-    l.ipiv = (int32_t*)(P.ws.p + piv[i]);
-    l.perm = (int32_t*)(P.ws.p + prm[i]);
-    l.info = (int32_t*)(P.ws.p + inf) + i;
-    l.dinv = P.ws.p + dv[i];
+    l.ipiv = (int32_t*)(P.ws.p + D.piv[i]);
+    l.perm = (int32_t*)(P.ws.p + D.prm[i]);
+    l.info = (int32_t*)(P.ws.p + D.inf) + i;
+    l.dinv = P.ws.p + D.dv[i];
     lus.push_back(l);
     CopyDesc c{};
-    c.src = B[i]; c.lds = ldb[i]; c.dst = P.ws.p + bc[i]; c.ldd = n[i];
+    c.src = B[i]; c.lds = ldb[i]; c.dst = P.ws.p + D.bc[i]; c.ldd = n[i];
     c.rows = n[i]; c.cols = nrhs[i]; c.alpha = 1.0;
     cps.push_back(c);
     TrsmDesc t{};
     t.LU = A[i]; t.ldlu = lda[i]; t.dinv = l.dinv; t.perm = l.perm;
-    t.Xin = P.ws.p + bc[i]; t.ldin = n[i]; t.Xout = B[i]; t.ldout = ldb[i];
+    t.Xin = P.ws.p + D.bc[i]; t.ldin = n[i]; t.Xout = B[i]; t.ldout = ldb[i];
     t.m = n[i]; t.ncols = nrhs[i];
     trs.push_back(t);
   }
   Bd.lu(pr, CAT_OTHER, CAT_OTHER, CAT_OTHER, CAT_OTHER, F64, lus);
   Bd.copy(pr, CAT_OTHER, F64, F64, cps);
   Bd.trsm(pr, CAT_OTHER, F64, F64, F64, trs);
-  int rc = run_oneoff(P, Bd, pr);
-  if (!rc && info && hipMemcpy(info, P.ws.p + inf, size_t(batch) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+}
+
+int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
+                        double* const* B, const int64_t* ldb, int32_t* info, void* stream) {
+  int rc = dgesv_check(batch, n, nrhs, A, lda, B, ldb);
+  if (rc) return rc;
+  hodlr_plan_s P;
+  P.stream = (hipStream_t)stream;
+  const DgesvLayout D = dgesv_layout(batch, n, nrhs);
+  if (hipMalloc((void**)&P.ws.p, D.bytes) != hipSuccess) return HODLR_ERR_NOMEM;
+  P.ws.bytes = D.bytes;
+  Builder Bd{&P};
+  Proc pr;
+  dgesv_build(P, Bd, pr, D, batch, n, nrhs, A, lda, B, ldb);
+  rc = run_oneoff(P, Bd, pr);
+  if (!rc && info && hipMemcpy(info, P.ws.p + D.inf, size_t(batch) * 4, hipMemcpyDeviceToHost) != hipSuccess)
     rc = HODLR_ERR_HIP;
   (void)hipFree(P.ws.p);
   P.ws.p = nullptr;
@@ -1420,67 +1474,17 @@ int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, do
 int hodlr_matvec(hodlr_plan_t P, const double* const* x, const int64_t* ldx, double* const* y,
                  const int64_t* ldy) {
   if (!P || !x || !ldx || !y || !ldy) return HODLR_ERR_ARG;
-  // y = A x via the residual pipeline with b = 0: build a one-off batch
-  Builder B{P};
-  Proc pr;
-  std::vector<Builder::G> red, lk, lu;
-  std::vector<SumDesc> wsum;
-  std::vector<CopyDesc> gz;
   for (size_t s = 0; s < P->sys.size(); ++s) {
     SysState& S = P->sys[s];
-    const Tree& t = S.tree;
-    const int64_t c = t.nrhs;
-    double* Z = at<double>(P, S.zbuf);
-    matvec_reduce(P, S, x[s], ldx[s], red, wsum);
-    for (size_t j = 0; j < t.leaves.size(); ++j) {
-      const int id = t.leaves[j];
-      const TreeNode& nd = t.nodes[id];
-      Builder::G g{};
-      g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : nd.nrow;
-      g.B = x[s] + nd.row_beg; g.ldb = ldx[s];
-      g.Cout = y[s] + nd.row_beg; g.ldcout = ldy[s];
-      g.m = int(nd.nrow); g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
-      lk.push_back(g);
-      const int64_t Rd = t.rsum[nd.depth];
-      if (Rd == 0) continue;
-      int a = id;
-      while (a != 0) {
-        const TreeNode& an = t.nodes[a];
-        const TreeNode& par = t.nodes[an.parent];
-        const int sib = par.child[0] == a ? par.child[1] : par.child[0];
-        const int r = t.rank(an.depth);
-        CopyDesc cd{};
-        cd.src = at<double>(P, S.wmv[sib]); cd.lds = r;
-        cd.dst = Z + S.z_off[j] + t.rsum[an.depth - 1]; cd.ldd = Rd;
-        cd.rows = r; cd.cols = int(c); cd.alpha = 1.0;
-        gz.push_back(cd);
-        a = an.parent;
-      }
-      Builder::G u{};
-      u.A = S.s.X + nd.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
-      u.B = Z + S.z_off[j]; u.ldb = Rd;
-      u.Cin = y[s] + nd.row_beg; u.ldcin = ldy[s]; u.Cout = y[s] + nd.row_beg; u.ldcout = ldy[s];
-      u.m = int(nd.nrow); u.n = int(c); u.k = int(Rd); u.alpha = 1.0; u.beta = 1.0;
-      lu.push_back(u);
-    }
+    if (!x[s] || !y[s] || ldx[s] < S.tree.n || ldy[s] < S.tree.n) return HODLR_ERR_ARG;
+    const size_t w = size_t(S.tree.n) * 8;
+    if (hipMemcpy2DAsync(P->ws.p + S.mvx, S.ldw * 8, x[s], ldx[s] * 8, w, S.tree.nrhs, hipMemcpyDeviceToDevice,
+                         P->stream) != hipSuccess)
+      return HODLR_ERR_HIP;
   }
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
-  B.sum(pr, CAT_MATVEC, wsum);
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
-  B.copy(pr, CAT_MATVEC, F64, F64, gz);
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
-  if (B.scratch_need > P->scratch_doubles) return HODLR_ERR_UNSUPPORTED;
-  void* tmp = nullptr;
-  if (hipMalloc(&tmp, B.blob.data.size()) != hipSuccess) return HODLR_ERR_NOMEM;
-  hipMemcpyAsync(tmp, B.blob.data.data(), B.blob.data.size(), hipMemcpyHostToDevice, P->stream);
-  char* save = P->descs.p;
-  P->descs.p = (char*)tmp;
-  int rc = run(P, pr);
-  P->descs.p = save;
-  hipError_t e = hipStreamSynchronize(P->stream);
-  hipFree(tmp);
-  if (rc) return rc;
-  return e == hipSuccess ? HODLR_OK : HODLR_ERR_HIP;
+  int rc = run(P, P->matvec);                       // rbuf <- A mvx
+  if (!rc) rc = stage_copy(P, nullptr, nullptr, false, y, ldy);
+  return rc;
 }
 
 int hodlr_residual(hodlr_plan_t P, double* relres, double* relres_max) {
@@ -1581,7 +1585,7 @@ int hodlr_autotune(hodlr_plan_t P, int32_t reps, int32_t* switched) {
     sw += best;
   }
   if (switched) *switched = sw;
-  if (P->graph_exec) { (void)hipGraphExecDestroy(P->graph_exec); P->graph_exec = nullptr; }
+  drop_graph(P);
   P->factored = false;   // the tuning runs left the workspace in an arbitrary state
   return HODLR_OK;
 }
@@ -1589,12 +1593,14 @@ int hodlr_autotune(hodlr_plan_t P, int32_t reps, int32_t* switched) {
 int hodlr_tune_force(hodlr_plan_t P, int32_t cfg) {
   if (!P || cfg < -1 || cfg > 1) return HODLR_ERR_ARG;
   for (auto& c : P->tile_choice) c = cfg;
-  if (P->graph_exec) { (void)hipGraphExecDestroy(P->graph_exec); P->graph_exec = nullptr; }
+  drop_graph(P);
   return HODLR_OK;
 }
 
 int hodlr_profile_enable(hodlr_plan_t P, int32_t mask) {
   if (!P) return HODLR_ERR_ARG;
+  // a captured graph holds the event pairs of the mask it was captured with
+  if (uint32_t(mask) != P->prof_mask) drop_graph(P);
   P->prof_mask = uint32_t(mask);
   return HODLR_OK;
 }

@@ -9,12 +9,13 @@ runtime (``DeviceArray``); numpy holds the host side.  Column-major storage:
 an N x k array has leading dimension N.
 """
 import ctypes
+import time
 from typing import List, Sequence
 
 import numpy as np
 
 from . import gen as _gen
-from ._lib import PlanInfo, SolveReport, System, check, device_lib
+from ._lib import PlanInfo, SolveReport, System, check, device_lib, gen_lib
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -53,9 +54,17 @@ class DeviceArray:
         check(device_lib().hodlr_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0), "upload")
 
     def download(self) -> np.ndarray:
+        """Host copy (hodlr_memcpy waits for every stream of the device first)."""
         out = np.empty((self.rows, self.cols), dtype=self.dtype, order="F")
         check(device_lib().hodlr_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1), "download")
         return out
+
+    def upload_cols(self, a: np.ndarray, col0: int):
+        """Write host columns a (rows x k) into columns col0 .. col0+k-1."""
+        a = np.asfortranarray(a, dtype=self.dtype)
+        assert a.shape[0] == self.rows and col0 + a.shape[1] <= self.cols
+        off = col0 * self.rows * self.dtype.itemsize
+        check(device_lib().hodlr_memcpy(self.ptr + off, a.ctypes.data, a.nbytes, 0), "upload")
 
     def free(self):
         if self.ptr:
@@ -293,24 +302,47 @@ class HodlrMatrix:
         self.file_rhs = name + "_rhs.txt"
         self.file_soln = name + "_soln.txt"
         self.spec = None
-        self.host = None
         self.dev = None
         self.seed = None
+        self.timeInit = 0.0
 
     def create_tree(self):
-        # the extra rank*(gl-sl) columns of a two-stage run (hodlr_matrix.cc:48) are not used on one GPU
+        """Tree and device arrays (hodlr_matrix.cc:44-82); the extra rank*(gl-sl)
+        columns of a two-stage run (hodlr_matrix.cc:48) are not used on one GPU."""
         self.spec = _gen.SystemSpec(self.rhs_rows, self.threshold, self.rhs_cols, [self.rank], kind="circulant")
-        _gen.tree_info(self.spec)
+        nl, md, xc, vc, kp = _gen.tree_info(self.spec)
+        self.dev = DeviceSystem(self.spec, DeviceArray(self.rhs_rows, xc), DeviceArray(self.rhs_rows, max(vc, 1)),
+                                DeviceArray(kp))
 
     def init_rhs(self, seed, procs=None):
+        """drand48 right-hand side reseeded per leaf region (hodlr_matrix.cc:175-209),
+        written into the first rhs_cols columns of X; either order with
+        init_circulant_matrix."""
+        t0 = time.perf_counter()
         self.seed = seed
         self.spec.seed = seed
+        b = np.empty((self.rhs_rows, self.rhs_cols), order="F")
+        rc = gen_lib().hodlr_gen_rhs_drand48(self.rhs_rows, self.threshold, self.rhs_cols, self.rank, seed, 0,
+                                             b.ctypes.data, self.rhs_rows)
+        if rc:
+            raise RuntimeError(f"hodlr_gen_rhs_drand48 failed: {rc}")
+        self.dev.X.upload_cols(b, 0)
+        self.timeInit += time.perf_counter() - t0
 
     def init_circulant_matrix(self, diag, procs=None, skipU=False):
+        """A = diag*I + U U^T, U(i,j) = (i+j) mod rank (hodlr_matrix.cc:94-109): U
+        into X's columns rhs_cols.. (unless skipU), V and K; b is left alone."""
+        t0 = time.perf_counter()
         self.spec.diag = diag
         X, V, K = _gen.generate(self.spec)
-        self.host = (X, V, K)
-        self.dev = DeviceSystem.from_host(self.spec, X, V, K)
+        if not skipU:
+            self.dev.X.upload_cols(X[:, self.rhs_cols:], self.rhs_cols)
+        self.dev.V.upload(V)
+        self.dev.K.upload(K)
+        self.timeInit += time.perf_counter() - t0
+
+    def display_launch_time(self):
+        print(f"Time cost for launching init-tasks :{self.timeInit} s")
 
     def get_num_leaf(self):
         return _gen.tree_info(self.spec)[0]
@@ -324,8 +356,11 @@ class HodlrMatrix:
     def solution(self):
         return self.dev.solution()
 
+    def rhs(self):
+        return self.dev.X.download()[:, : self.rhs_cols]
+
     def save_rhs(self):
-        _save_rows(self.file_rhs, self.host[0][:, : self.rhs_cols], self.spec, seed=self.seed)
+        _save_rows(self.file_rhs, self.rhs(), self.spec, seed=self.seed)
 
     def save_solution(self):
         _save_rows(self.file_soln, self.solution(), self.spec)
@@ -351,9 +386,15 @@ class FastSolver:
         self.precision = precision
         self.plan = None
 
-    def bfs_solve(self, hmat: HodlrMatrix, procs=None):
+    def bfs_solve(self, hmat: HodlrMatrix, procs=None, tol=1e-12, maxiter=20):
+        """Factor + solve; the result is in hmat's solution array when this returns
+        (FP32/BF16 plans refine in FP64 to `tol`)."""
         self.plan = Plan([hmat.dev], self.precision)
-        self.plan.factor_solve()
+        if self.precision == PREC_FP64:
+            self.plan.factor_solve()
+        else:
+            self.report = self.plan.solve(tol, maxiter)
+        self.plan.synchronize()
         return self.plan
 
 

@@ -9,6 +9,10 @@
 //   init_circulant_matrix(diag, range, ctx, rt, skipU) init_circulant_matrix(diag, range, skipU)
 //   save_rhs / save_solution                          same (text format of SaveRegionTask)
 //   launch_level, get_num_rhs, get_num_leaf, get_file_soln   same
+//   display_launch_time()                             same: host time of init_rhs + init_circulant_matrix
+//   Node* uroot, vroot (hodlr_matrix.h:49-50)         same names; Node (include/htree/node.h:12-43) keeps
+//                                                     nrow/ncol/row_beg/col_beg/lchild/rchild, and the
+//                                                     node's device block (ptr, ld) replaces its LMatrix*
 //   FastSolver::bfs_solve(hmat, range, ctx, rt)       bfs_solve(hmat, range) (fast_solver.h:19-20)
 //   register_solver_tasks()                           register_solver_tasks(): checks a GPU is present
 //   compute_L2_error(hmat, seed, rows, nregions,      same minus ctx/rt (include/solver/direct_solve.h:20-24)
@@ -22,7 +26,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <chrono>
 #include <cmath>
+#include <memory>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -50,6 +56,22 @@ class Range {
   Range lchild() const { return Range(mbegin, msize / 2); }
   Range rchild() const { return Range(mbegin + msize / 2, msize / 2); }
   int mbegin, msize;
+};
+
+// Tree node of the reference (include/htree/node.h:12-43).  The U tree has
+// the reference's column layout (root: the c right-hand-side columns; a child
+// of a depth-(d-1) node: `rank` columns at col_beg = c + (d-1)*rank of X); the
+// V tree's blocks sit at col_beg = (d-1)*rank of V.  Where the reference keeps
+// an LMatrix (a Legion region) per node, `ptr` points at the node's nrow x ncol
+// block in device memory (leading dimension `ld`).  legion leaf = real leaf.
+struct Node {
+  int nrow = 0, ncol = 0, row_beg = 0, col_beg = 0;
+  Node* lchild = nullptr;
+  Node* rchild = nullptr;
+  double* ptr = nullptr;
+  int64_t ld = 0;
+  bool is_real_leaf() const { return lchild == nullptr && rchild == nullptr; }
+  bool is_legion_leaf() const { return is_real_leaf(); }
 };
 
 // Device buffer owned by the facade.
@@ -102,24 +124,56 @@ class HodlrMatrix {
     V = DeviceBuffer(int64_t(rhs_rows) * std::max<int64_t>(vcols, 1) * 8);
     K = DeviceBuffer(kpacked * 8);
     x = DeviceBuffer(int64_t(rhs_rows) * rhs_cols * 8);
+    nodes.clear();
+    uroot = new_node(rhs_rows, rhs_cols, 0, 0, (double*)X.get());
+    vroot = new_node(rhs_rows, 0, 0, 0, (double*)V.get());
+    split(uroot, vroot, 1);
   }
 
-  // The reseeded drand48 right-hand side (hodlr_matrix.cc:175-209).  The data is
-  // generated together with the matrix by init_circulant_matrix.
-  void init_rhs(long seed, const Range& = Range()) { rhs_seed = seed; }
+  // The drand48 right-hand side, reseeded with `seed` at the start of every
+  // leaf region and drawn row by row (hodlr_matrix.cc:175-209,
+  // init_matrix_tasks.cc:43-77): generated on the host and written into the
+  // first rhs_cols columns of X.  Independent of init_circulant_matrix, so the
+  // two calls may come in either order.
+  void init_rhs(long seed, const Range& = Range()) {
+    const auto t0 = std::chrono::steady_clock::now();
+    rhs_seed = seed;
+    std::vector<double> hb(size_t(rhs_rows) * rhs_cols);
+    check(hodlr_gen_rhs_drand48(rhs_rows, threshold, rhs_cols, rank, seed, 0, hb.data(), rhs_rows), "init_rhs");
+    X.upload(hb.data(), int64_t(hb.size()) * 8);   // ld = rhs_rows: the first rhs_cols columns are contiguous
+    timeInit += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
 
-  // A = diag*I + U U^T with U(i,j) = (i+j) mod rank (hodlr_matrix.cc:94-109).
-  void init_circulant_matrix(double diag, const Range& = Range(), bool /*skipU*/ = false) {
+  // A = diag*I + U U^T with U(i,j) = (i+j) mod rank (hodlr_matrix.cc:94-109):
+  // U into X's columns rhs_cols.. (skipped with skipU, as the reference's
+  // init_Umat), V, and the leaf blocks K.  The right-hand-side columns are
+  // left as init_rhs wrote them.
+  void init_circulant_matrix(double diag, const Range& = Range(), bool skipU = false) {
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<double> hX(size_t(rhs_rows) * ncolX);
     std::vector<double> hV(size_t(rhs_rows) * std::max<int64_t>(ncolV, 1));
     std::vector<double> hK(static_cast<size_t>(nK));
     check(hodlr_gen_circulant(rhs_rows, threshold, rhs_cols, rank, diag, rhs_seed, 0, hX.data(), rhs_rows,
                               hV.data(), rhs_rows, hK.data(), 0, HODLR_K_PACKED),
           "init_circulant_matrix");
-    X.upload(hX.data(), int64_t(hX.size()) * 8);
+    const size_t off = size_t(rhs_rows) * rhs_cols;
+    if (!skipU)
+      check(hodlr_memcpy((double*)X.get() + off, hX.data() + off, int64_t(hX.size() - off) * 8, 0),
+            "init_circulant_matrix");
     V.upload(hV.data(), int64_t(hV.size()) * 8);
     K.upload(hK.data(), int64_t(hK.size()) * 8);
+    timeInit += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+
+  // hodlr_matrix.h:43-46: the time spent initialising the matrix and the
+  // right-hand side (host generation and upload here; task launches there).
+  void display_launch_time() const {
+    std::cout << "Time cost for launching init-tasks :" << timeInit << " s" << std::endl;
+  }
+
+  // Tree roots (hodlr_matrix.h:49-50), valid after create_tree.
+  Node* uroot = nullptr;
+  Node* vroot = nullptr;
 
   // Text dumps in SaveRegionTask's format (save_region_task.cc:37-67): per leaf
   // region, optionally the seed, then one row per line of tab-terminated values.
@@ -180,6 +234,33 @@ class HodlrMatrix {
     }
   }
 
+  Node* new_node(int nrow, int ncol, int row_beg, int col_beg, double* base) {
+    nodes.emplace_back(new Node());
+    Node* p = nodes.back().get();
+    p->nrow = nrow;
+    p->ncol = ncol;
+    p->row_beg = row_beg;
+    p->col_beg = col_beg;
+    p->ld = rhs_rows;
+    p->ptr = base + row_beg + int64_t(col_beg) * rhs_rows;
+    return p;
+  }
+  // split rule of create_balanced_tree (hodlr_matrix.cc:138-169): N/2 | N - N/2
+  // while nrow > threshold; children of a depth-(d-1) node sit at depth d
+  void split(Node* u, Node* v, int d) {
+    if (u->nrow <= threshold) return;
+    const int n0 = u->nrow / 2, n1 = u->nrow - n0;
+    const int ucol = u->col_beg + u->ncol, vcol = (d - 1) * rank;
+    u->lchild = new_node(n0, rank, u->row_beg, ucol, (double*)X.get());
+    u->rchild = new_node(n1, rank, u->row_beg + n0, ucol, (double*)X.get());
+    v->lchild = new_node(n0, rank, v->row_beg, vcol, (double*)V.get());
+    v->rchild = new_node(n1, rank, v->row_beg + n0, vcol, (double*)V.get());
+    split(u->lchild, v->lchild, d + 1);
+    split(u->rchild, v->rchild, d + 1);
+  }
+
+  std::vector<std::unique_ptr<Node>> nodes;
+  double timeInit = 0;
   int rhs_cols = 0, rhs_rows = 0, gloLevel = 0, subLevel = 0;
   int32_t rank = 0;
   int threshold = 0, leafSize = 1, nLegionLeaf = 0, depth = 0;

@@ -52,6 +52,12 @@ int hodlr_gen_circulant(int64_t n, int64_t threshold, int nrhs, int rank, double
                         int64_t rhs_region_rows, double* X, int64_t ldx, double* V, int64_t ldv, double* K,
                         int64_t ldk, int klayout);
 
+/* The reference's right-hand side alone (HodlrMatrix::init_rhs, hodlr_matrix.cc:175-209):
+ * columns 0 .. nrhs-1 of X, drand48 reseeded with `seed` at the start of every
+ * region (the leaves when rhs_region_rows <= 0), drawn row by row. */
+int hodlr_gen_rhs_drand48(int64_t n, int64_t threshold, int nrhs, int rank, int64_t seed,
+                          int64_t rhs_region_rows, double* X, int64_t ldx);
+
 /* Seeded random family (right-hand side uniform [0,1)). */
 int hodlr_gen_random(int64_t n, int64_t threshold, int nrhs, int nranks, const int* ranks,
                      const hodlr_family* fam, double* X, int64_t ldx, double* V, int64_t ldv, double* K,

@@ -34,6 +34,7 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.oracle_lapack_name.argtypes = [ctypes.c_char_p, _int]
+        L.oracle_lapack_version.argtypes = [ctypes.c_char_p, _int]
         L.oracle_set_blas_threads.argtypes = [_int]
         L.oracle_set_blas_threads.restype = None
         L.oracle_hodlr_solve.argtypes = [_i64, _i64, _int, _int, _ptr, _ptr, _i64, _ptr, _i64, _ptr, _i64, _int,
@@ -70,6 +71,13 @@ def lapack_name():
     if lib().oracle_lapack_name(buf, 512) != 0:
         raise RuntimeError("no LAPACK found for the oracle")
     return buf.value.decode()
+
+
+def lapack_version():
+    buf = ctypes.create_string_buffer(512)
+    if lib().oracle_lapack_version(buf, 512) != 0:
+        raise RuntimeError("no LAPACK found for the oracle")
+    return buf.value.decode().strip()
 
 
 def set_blas_threads(n):

@@ -302,6 +302,25 @@ int oracle_lapack_name(char* buf, int len) {
   return 0;
 }
 
+// Version string of the loaded LAPACK (MKL_Get_Version_String, or OpenBLAS's
+// get_config); the library path when neither is exported.
+int oracle_lapack_version(char* buf, int len) {
+  if (!lapack()) return -1;
+  typedef void (*mkl_ver_t)(char*, int);
+  typedef const char* (*ob_cfg_t)();
+  if (void* f = dlsym(g_lp.h, "MKL_Get_Version_String")) {
+    ((mkl_ver_t)f)(buf, len);
+    return 0;
+  }
+  for (const char* sym : {"scipy_openblas_get_config64_", "scipy_openblas_get_config", "openblas_get_config"})
+    if (void* f = dlsym(g_lp.h, sym)) {
+      snprintf(buf, len, "%s", ((ob_cfg_t)f)());
+      return 0;
+    }
+  snprintf(buf, len, "%s", g_lp.name.c_str());
+  return 0;
+}
+
 // Threads for the BLAS inside one call (MKL only).
 void oracle_set_blas_threads(int n) {
   if (lapack() && g_lp.set_threads) g_lp.set_threads(n);

@@ -4,7 +4,6 @@ import numpy as np
 
 import oracle
 from fastsolver_amd import gen
-from fastsolver_amd import hodlr
 
 
 def host_system(spec):
@@ -19,10 +18,6 @@ def oracle_solve(spec, X, V, K, **kw):
     rc = oracle.hodlr_solve(spec.as_dict(), Xo, V, Ko, **kw)
     assert rc == 0, rc
     return Xo
-
-
-def device_system(spec, X, V, K):
-    return hodlr.DeviceSystem.from_host(spec, X, V, K, "cuda")
 
 
 def rel(a, b):

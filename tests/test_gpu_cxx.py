@@ -26,13 +26,21 @@ def _read_regions(path):
     return np.array(rows)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_single_launch_driver(tmp_path, precision):
-    print(f"single_launch precision={precision}", flush=True)
-    p = subprocess.run([EXE, str(precision)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+@pytest.mark.parametrize("precision,rhs_last", [(0, 0), (1, 0), (0, 1), (2, 1)])
+def test_single_launch_driver(tmp_path, precision, rhs_last):
+    """test/single_launch.cc:42-75's call sequence (create_tree, init_rhs,
+    init_circulant_matrix, FastSolver, bfs_solve, display_launch_time of both,
+    compute_L2_error); rhs_last swaps the two init calls."""
+    print(f"single_launch precision={precision} rhs_last={rhs_last}", flush=True)
+    p = subprocess.run([EXE, str(precision), str(rhs_last)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0, (p.returncode, p.stderr)
     assert "Residual:" in p.stdout
+    assert "Time cost for launching init-tasks :" in p.stdout
+    # the tree accessor: N = 480 over 3 levels, leaves of 60 rows, depth-3 U block at column 2 + 2*30
+    assert "uroot: 480 x 2, 3 levels; leftmost leaf 60 rows, U block at column 62; vroot depth-1 block at V column 0" \
+        in p.stdout
     # the reference's own error report (compute_L2_error), computed by the facade
     err = float(p.stdout.split("Err:")[1].split()[0])
     assert err <= 1e-12, err
@@ -44,3 +52,24 @@ def test_single_launch_driver(tmp_path, precision):
     x = _read_regions(tmp_path / "global_soln.txt")
     assert x.shape == (spec.n, spec.nrhs)
     assert rel(x, xw) <= 1e-12
+
+
+def test_python_facade_bfs_solve():
+    """The Python mirror of the same driver: bfs_solve leaves the solution ready
+    for save_solution / solution() (the plan stream is synchronised)."""
+    from fastsolver_amd import hodlr
+    spec = gen.single_launch_case()
+    for order in (0, 1):
+        h = hodlr.HodlrMatrix(2, 480, 3, 3, 30, 60, 1, "py")
+        h.create_tree()
+        if order == 0:
+            h.init_rhs(1245667)
+        h.init_circulant_matrix(1.0e4)
+        if order == 1:
+            h.init_rhs(1245667)
+        fs = hodlr.FastSolver(hodlr.PREC_FP64)
+        fs.bfs_solve(h)
+        X, V, K = host_system(spec)
+        xw = oracle.woodbury_circulant(spec.n, spec.ranks[0], spec.diag, X[:, :spec.nrhs].copy(order="F"))
+        assert rel(h.solution(), xw) <= 1e-12
+        assert np.array_equal(h.rhs(), X[:, :spec.nrhs])

@@ -75,7 +75,7 @@ def test_random_families_fp64(cfg, scale):
     assert e <= TOL
     assert relres <= TOL
     W = plan.workspace(0)
-    assert colblock_rel(W, Xo, _factor_blocks(spec)) <= 1e-11
+    assert colblock_rel(W, Xo, _factor_blocks(spec)) <= TOL
 
 
 def test_phases_match_fused_pass():
@@ -140,7 +140,7 @@ def test_fp32_factor_with_fp64_refinement():
     print(f"  fp32: first relres {rep.relres_first:.3e}, {rep.iterations} sweeps, relres {rep.relres:.3e}", flush=True)
     assert rep.converged and rep.relres <= TOL
     assert rep.relres_first < 1e-4
-    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
 
 
 @pytest.mark.parametrize("cfg,scale", [(1, 4), (3, 32)])
@@ -156,7 +156,7 @@ def test_bf16_factor_with_fp64_refinement(cfg, scale):
     print(f"  bf16: first relres {rep.relres_first:.3e}, {rep.iterations} sweeps, relres {rep.relres:.3e}", flush=True)
     assert rep.converged and rep.relres <= TOL
     assert 1e-6 < rep.relres_first < 1e-1      # really a low-precision first solve
-    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
 
 
 def test_uneven_batch_one_plan():
@@ -235,7 +235,7 @@ def test_every_kernel_variant(cfg, prec):
     plan.force_tiling(cfg)
     rep = plan.solve(tol=TOL, maxiter=30)
     assert rep.converged
-    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= (TOL if prec == hodlr.PREC_FP64 else 1e-11)
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
 
 
 @pytest.mark.parametrize("prec", [hodlr.PREC_FP64, hodlr.PREC_FP32])
@@ -250,7 +250,7 @@ def test_single_leaf_system(prec):
     assert plan.info().height == 0
     rep = plan.solve(tol=TOL, maxiter=10)
     assert rep.converged
-    assert rel(ds.solution(), Xo[:, :3]) <= (TOL if prec == hodlr.PREC_FP64 else 1e-11)
+    assert rel(ds.solution(), Xo[:, :3]) <= TOL
 
 
 @pytest.mark.parametrize("prec", [hodlr.PREC_FP32, hodlr.PREC_BF16])
@@ -265,30 +265,34 @@ def test_cfg3_full_size_mixed(prec):
     rep = plan.solve(tol=TOL, maxiter=30)
     print(f"  prec {prec}: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}", flush=True)
     assert rep.converged and rep.relres <= TOL
-    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= 1e-11
+    assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
 
 
 @pytest.mark.slow
-def test_cfg5_full_size_residuals():
+def test_cfg5_full_size_against_oracle():
     """N = 2^20, 10 levels, rank 128, 32 rhs at full size: every precision reaches
-    a relative residual <= 1e-12 (FP64 in one pass, FP32/BF16 after refinement),
-    the three solutions agree, and a second factor+solve reproduces the first
+    a relative residual <= 1e-12 (FP64 in one pass, FP32/BF16 after refinement)
+    and a solution within 1e-12 of the oracle's (the reference's task bodies over
+    host LAPACK, all 32 columns), and a second factor+solve reproduces the first
     bitwise (fixed-order reductions)."""
     spec = gen.config(5)[0]
     X, V, K = host_system(spec)
     ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    print("  inputs uploaded; oracle solve on the host ...", flush=True)
+    xo = oracle_solve(spec, X, V, K, order=1, nthreads=16)[:, :spec.nrhs].copy(order="F")
     del X, V, K
-    sols = {}
     for prec in (hodlr.PREC_FP64, hodlr.PREC_FP32, hodlr.PREC_BF16):
         plan = hodlr.Plan([ds], prec)
         rep = plan.solve(tol=TOL, maxiter=30)
-        print(f"  prec {prec}: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}",
-              flush=True)
+        plan.synchronize()
+        x = ds.solution()
+        e = rel(x, xo)
+        print(f"  prec {prec}: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}, "
+              f"err vs oracle {e:.2e}", flush=True)
         assert rep.converged and rep.relres <= TOL
-        sols[prec] = ds.solution()
+        assert e <= TOL
         if prec == hodlr.PREC_FP64:
             plan.factor_solve()
-            assert np.array_equal(ds.solution(), sols[prec])
+            plan.synchronize()
+            assert np.array_equal(ds.solution(), x)
         plan.close()
-    assert rel(sols[hodlr.PREC_FP32], sols[hodlr.PREC_FP64]) <= 1e-11
-    assert rel(sols[hodlr.PREC_BF16], sols[hodlr.PREC_FP64]) <= 1e-11
