@@ -67,6 +67,7 @@ DEVICE_SYMBOLS = {
     "hodlr_sgemm_batched_bf16": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr],
                             ctypes.c_int),
     "hodlr_dgesv_batched": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
+    "hodlr_dgesv_batched_noalloc": ([_i32, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
     "hodlr_device_count": ([ctypes.POINTER(_i32)], ctypes.c_int),
     "hodlr_device_name": ([ctypes.c_char_p, _i32], ctypes.c_int),
     "hodlr_device_malloc": ([ctypes.POINTER(_ptr), _i64], ctypes.c_int),

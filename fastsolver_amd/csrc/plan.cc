@@ -25,6 +25,7 @@
 #include <cmath>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1137,6 +1138,37 @@ int read_relres(hodlr_plan_s* P, double* relres, double* mx) {
   return HODLR_OK;
 }
 
+// ---------------------------------------------------- fixed scratch buffer
+// hodlr_dgesv_batched_noalloc keeps its descriptors and workspace in this
+// file-scope device array instead of memory it allocates: one copy per GPU,
+// held until the process ends.  Host code never uses the array's name as an
+// address; it asks the runtime for this device's copy once and keeps it.
+constexpr size_t kScratchBytes = size_t(64) << 20;      // 64 MiB per device
+constexpr size_t kScratchDescBytes = size_t(4) << 20;   // of which the descriptor tables
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+// Device address of `scratch` on the current device (looked up on its first
+// call on that device).
+int scratch_on_current_device(char** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return HODLR_ERR_HIP;
+  if (dev < 0 || dev >= kScratchMaxDevices) return HODLR_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HODLR_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (char*)g_scratch_ptr[dev];
+  return HODLR_OK;
+}
 }  // namespace
 
 // ======================================================================= C-ABI
@@ -1154,6 +1186,7 @@ const char* hodlr_error_string(int32_t code) {
     case HODLR_ERR_NOMEM: return "device allocation failed";
     case HODLR_ERR_SINGULAR: return "zero pivot in a leaf block or a Schur complement";
     case HODLR_ERR_NOTCONVERGED: return "refinement did not reach the tolerance";
+    case HODLR_ERR_SCRATCH: return "input exceeds the fixed scratch buffer of the no-allocation entry point";
   }
   return "unknown error";
 }
@@ -1468,6 +1501,38 @@ int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, do
     rc = HODLR_ERR_HIP;
   (void)hipFree(P.ws.p);
   P.ws.p = nullptr;
+  return rc;
+}
+
+int hodlr_dgesv_batched_noalloc(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A,
+                                const int64_t* lda, double* const* B, const int64_t* ldb, int32_t* info,
+                                void* stream) {
+  int rc = dgesv_check(batch, n, nrhs, A, lda, B, ldb);
+  if (rc) return rc;
+  const DgesvLayout D = dgesv_layout(batch, n, nrhs);
+  if (D.bytes > kScratchBytes - kScratchDescBytes) return HODLR_ERR_SCRATCH;
+  char* base = nullptr;
+  if ((rc = scratch_on_current_device(&base))) return rc;
+  hodlr_plan_s P;
+  P.stream = (hipStream_t)stream;
+  P.ws.p = base + kScratchDescBytes;   // workspace after the descriptor region
+  P.ws.bytes = D.bytes;
+  Builder Bd{&P};
+  Proc pr;
+  dgesv_build(P, Bd, pr, D, batch, n, nrhs, A, lda, B, ldb);
+  if (Bd.blob.data.size() > kScratchDescBytes) return HODLR_ERR_SCRATCH;
+  P.descs.p = base;
+  if (hipMemcpyAsync(base, Bd.blob.data.data(), Bd.blob.data.size(), hipMemcpyHostToDevice, P.stream) != hipSuccess)
+    return HODLR_ERR_HIP;
+  rc = run(&P, pr);
+  if (!rc && info &&
+      hipMemcpyAsync(info, P.ws.p + D.inf, size_t(batch) * 4, hipMemcpyDeviceToHost, P.stream) != hipSuccess)
+    rc = HODLR_ERR_HIP;
+  // the descriptor blob must stay alive until its upload is done, and info is
+  // valid on return
+  if (hipStreamSynchronize(P.stream) != hipSuccess && !rc) rc = HODLR_ERR_HIP;
+  P.ws.p = nullptr;
+  P.descs.p = nullptr;
   return rc;
 }
 

@@ -45,6 +45,7 @@ extern "C" {
 #define HODLR_ERR_NOMEM -5       /* device allocation failed */
 #define HODLR_ERR_SINGULAR -6    /* a zero pivot was met (see hodlr_status) */
 #define HODLR_ERR_NOTCONVERGED -7
+#define HODLR_ERR_SCRATCH -8     /* input too large for the fixed scratch buffer (hodlr_dgesv_batched_noalloc) */
 
 typedef struct hodlr_plan_s* hodlr_plan_t;
 
@@ -167,6 +168,21 @@ int hodlr_sgemm_batched_bf16(int32_t batch, const int32_t* m, const int32_t* n, 
                              const int64_t* ldc, void* stream);
 int hodlr_dgesv_batched(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A, const int64_t* lda,
                         double* const* B, const int64_t* ldb, int32_t* info, void* stream);
+
+/* hodlr_dgesv_batched without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * pivots, permutations, inverted diagonal blocks, copy of B, info words and
+ * launch descriptors live in a fixed 64 MiB device array of the library (one
+ * per GPU, held until the process ends; 4 MiB of it for descriptors), so the
+ * workspace, sum over the batch of 8*(n*nrhs + 2*64*64*ceil(n/64)) + 8*n + 4
+ * bytes (each term rounded up to 256), must fit in 60 MiB: larger batches
+ * return HODLR_ERR_SCRATCH.  Results equal hodlr_dgesv_batched's.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (different
+ * streams or host threads) race.  Synchronises `stream`. */
+int hodlr_dgesv_batched_noalloc(int32_t batch, const int32_t* n, const int32_t* nrhs, double* const* A,
+                                const int64_t* lda, double* const* B, const int64_t* ldb, int32_t* info,
+                                void* stream);
 
 /* Workspace of system `sys`: N x (c + R) factor [x | A_sub^-1 U] (the
  * reference's U region after the solve), element type 0 = double, 1 = float. */
