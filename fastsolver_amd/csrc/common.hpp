@@ -63,7 +63,7 @@ template <class T> __device__ __forceinline__ HODLR_GLOBAL T* glob(T* p) { retur
 
 // Out-of-range lanes of a staged tile load from here: the value is already
 // zero, so no select (and no wait for the load) follows the load.
-static __device__ double g_zero_page[2];
+static __device__ __attribute__((aligned(16))) double g_zero_page[2];
 template <class T> __device__ __forceinline__ const HODLR_GLOBAL T* zero_page() {
   return (const HODLR_GLOBAL T*)g_zero_page;
 }

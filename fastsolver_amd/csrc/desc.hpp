@@ -58,6 +58,21 @@ struct TrsmDesc {
   void* Xout;
   int64_t ldout;
   int32_t m, ncols;
+  const void* LUt;      // bf16 plans: LU rounded to bf16 in MFMA A-fragment order (TransDesc),
+  int64_t ldt;          //   columns padded with zeros to ldt = m rounded up to 32
+  int64_t tile_begin;
+};
+
+// dst = bf16(src) (m x m, ld lds) in MFMA A-fragment order: 16 rows x 32
+// columns per 1 KB fragment, fragments row group by row group, ldd/32 per
+// group; zero past m (rows up to a multiple of 16, columns up to ldd).
+// One workgroup per 64 x 64 tile, tiles_k tiles along k per row band.
+struct TransDesc {
+  const void* src;
+  int64_t lds;
+  void* dst;
+  int64_t ldd;
+  int32_t m, tiles_k;
   int64_t tile_begin;
 };
 

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -820,6 +822,280 @@ __global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __res
   }
 }
 
+// ============================================================ bf16 LDS TRSM
+// bf16 plans: one workgroup per (block, CB-column panel) computes Y = L^-1 P X_in
+// and then X_out = U^-1 Y, 64 rows per step, with the panel's solved rows kept
+// in LDS as bf16 -- the operand precision of the products with solved rows --
+// so a step reads from memory only the factor's block row (from the
+// transposed bf16 copy: 16-byte k-contiguous fragments go straight into the
+// MFMA A registers, issued PF k-steps ahead) and its own right-hand-side rows.
+// The inverted 64 x 64 diagonal block is applied in fp32 (16x16x4 MFMA).
+//   s_y: solved rows, octet-planar: element (k, c) at s_y[((k/8)*CB + c)*8 + k%8]
+//   s_t: the step's 64 rows, fp32 [r][c]
+// Wave w owns rows 16w .. 16w+15 of a step and all CB columns.
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads stay in
+// flight across it (a full __syncthreads() waits for them).  The kernels that
+// use it exchange data between waves through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// bf16x3 product of fp32 operands: a*b ~ ah*bh + ah*bl + al*bh with the hi/lo
+// bf16 splits x = xh + xl (relative error ~2^-16, the small terms added first)
+__device__ __forceinline__ void split_bf16(const float* x, bf16x8& h, bf16x8& l) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const bf16 xh = bf16(x[e]);
+    h[e] = xh;
+    l[e] = bf16(x[e] - float(xh));
+  }
+}
+
+// Waves: 4 row groups of 16 rows x CB/32 column groups of 32 columns, so a
+// 64-column panel runs 8 waves (two per SIMD) and the waves of a SIMD hide
+// each other's latencies.
+template <class TIN, int MMAX, int CB>
+__global__ __launch_bounds__(CB * 8) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+  constexpr int NT = CB * 8;
+  // PF: factor fragments in flight per wave, enough for a whole step of
+  // the largest blocks; with the right-hand side and the inverse fragments
+  // the loads in flight stay under the 63 the wait counter can express
+  constexpr int RB = kDB, NJ = 2, PT = CB + 1, NQ = CB * RB / NT, ND = 16;
+  constexpr int PF = MMAX >= 1024 ? 24 : MMAX / 32;
+  static_assert(RB == 64 && (CB == 32 || CB == 64), "16-row x 32-column waves");
+  static_assert(PF + NQ + ND <= 63, "loads in flight");
+  __shared__ __attribute__((aligned(16))) bf16 s_y[MMAX * CB];
+  __shared__ float s_t[RB * PT];
+  __shared__ int s_perm[MMAX];   // the row gather of the forward pass, read once
+
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+  const TrsmDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int c0 = int(bid - d.tile_begin) * CB;
+  const int m = d.m, nc = min(CB, d.ncols - c0);
+  for (int i = threadIdx.x; i < m; i += NT) s_perm[i] = d.perm ? d.perm[i] : i;
+  lds_barrier();
+  const HODLR_GLOBAL bf16* LT = glob((const bf16*)d.LUt);
+  const int64_t ldt = d.ldt;
+  const HODLR_GLOBAL float* DI = glob((const float*)d.dinv);
+  const int nblk = (m + RB - 1) / RB;
+  const int kend = (m + 31) & ~31;
+  // (the wave index in a scalar register: every per-wave condition and
+  // fragment base below is then scalar)
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lq = lane >> 4;
+  const int wr = wave & 3, wc = 32 * (wave >> 2);   // the wave's row group and first column
+  // coalesced fill / write-back: thread owns row fr of columns fc + CS*q; the
+  // column bases are computed once (a column past nc reads the zero page)
+  const int fr = tid % RB, fc = __builtin_amdgcn_readfirstlane(tid / RB);
+  constexpr int CS = NT / RB;
+  const HODLR_GLOBAL TIN* xin_col[NQ];
+  HODLR_GLOBAL float* xout_col[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int c = min(fc + CS * q, nc - 1);
+    xin_col[q] = glob((const TIN*)d.Xin) + int64_t(c0 + c) * d.ldin;
+    xout_col[q] = glob((float*)d.Xout) + int64_t(c0 + c) * d.ldout;
+  }
+  const HODLR_GLOBAL bf16x8* zero8 = (const HODLR_GLOBAL bf16x8*)zero_page<double>();
+  const int wrow = 16 * wr + li;   // the thread's row of a step in the MFMA A layout
+
+  // per-step register set, reloaded in place for the next step as soon as
+  // its values are consumed (no second set, no copies)
+  bf16x8 ar[PF];         // the step's factor fragments
+  float dv[ND];          // the inverted diagonal block, A fragments (row wrow, k = 32kk + 8lq + e)
+  TIN xr[NQ];            // forward right-hand side (X_in rows gathered through perm)
+  float xb[NQ];          // backward right-hand side (the forward result)
+
+  auto blk = [&](int t) { return t < nblk ? t : 2 * nblk - 1 - t; };
+  // the factor's MFMA A fragment of rows 16g .. 16g+15, k 32ks .. 32ks+31:
+  // 1 KB, this lane's 16 bytes at `lane` (the copy is stored fragment by
+  // fragment, so every load is one contiguous 1 KB per wave)
+  const int nks = int(ldt >> 5);
+  auto frag = [&](int g, int ks) {
+    return (const HODLR_GLOBAL bf16x8*)LT + (int64_t(g) * nks + ks) * 64 + lane;
+  };
+  auto load_step = [&](int t) {        // factor fragments of step t's update
+    const int r0 = blk(t) * RB;
+    const int kb = t < nblk ? 0 : r0 + RB, ke = t < nblk ? r0 : kend;
+    const int nk = ke > kb ? (ke - kb) >> 5 : 0;
+    const bool ok = r0 + 16 * wr < m;
+    const HODLR_GLOBAL bf16x8* p = frag(ok ? (r0 >> 4) + wr : 0, kb >> 5);
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+      if (ok && u < nk) ar[u] = p[64 * u];
+  };
+  auto load_dinv = [&](int t) {
+    const HODLR_GLOBAL float* inv = DI + int64_t(blk(t)) * 2 * RB * RB + (t < nblk ? 0 : RB * RB) + wrow;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dv[kk * 8 + e] = inv[RB * (32 * kk + 8 * lq + e)];
+  };
+  auto load_rhs_fwd = [&](int t) {
+    const int r0 = blk(t) * RB, nr = min(RB, m - r0);
+    const int src = s_perm[min(r0 + fr, m - 1)];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xr[q] = *(fr < nr && fc + CS * q < nc ? xin_col[q] + src : zero_page<TIN>());
+  };
+  auto load_rhs_bwd = [&](int t) {
+    const int r0 = blk(t) * RB, nr = min(RB, m - r0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xb[q] = *(fr < nr && fc + CS * q < nc ? xout_col[q] + (r0 + fr) : zero_page<float>());
+  };
+
+  // One step: rows r0 .. r0+63 of the forward (t < nblk) or backward solve.
+  auto step = [&](int t, auto fwd) {
+    constexpr bool FWD = decltype(fwd)::value;
+    const int b = blk(t), r0 = b * RB, nr = min(RB, m - r0);
+    const int kb = FWD ? 0 : r0 + RB, ke = FWD ? r0 : kend;
+    const int nk = ke > kb ? (ke - kb) >> 5 : 0;
+    const bool rowok = r0 + 16 * wr < m;
+    const HODLR_GLOBAL bf16x8* arow = frag(rowok ? (r0 >> 4) + wr : 0, kb >> 5);
+    // update: acc = factor block row x solved rows (bf16 MFMA, fp32 sums);
+    // fragments beyond PF (only in the last steps of the largest blocks) are
+    // loaded into the slot just consumed
+    f32x4 acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[j] = f32x4{0, 0, 0, 0};
+    auto kstep = [&](int i, const bf16x8& a) {
+      const bf16* yb = s_y + ((((kb >> 3) + 4 * i + lq) * CB) + wc + li) * 8;
+      bf16x8 bv[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) bv[j] = *(const bf16x8*)(yb + 16 * j * 8);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) acc[j] = mfma_bf16(a, bv[j], acc[j]);
+    };
+    int i0 = 0;
+    for (; i0 + PF <= nk; i0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        kstep(i0 + u, ar[u]);
+        ar[u] = *(rowok && i0 + u + PF < nk ? arow + 64 * (i0 + u + PF) : zero8);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+      if (i0 + u < nk) kstep(i0 + u, ar[u]);
+    // T = rhs - update (the first backward step's rows are the last forward
+    // step's output, still in s_t); the freed registers take the next step's
+    // loads, which arrive while this step finishes
+    if (FWD) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) s_t[fr * PT + fc + CS * q] = float(xr[q]);
+      if (t + 1 < nblk) load_rhs_fwd(t + 1);
+    } else {
+      if (t != nblk) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) s_t[fr * PT + fc + CS * q] = xb[q];
+      }
+      if (t + 1 < 2 * nblk) load_rhs_bwd(t + 1);
+    }
+    if (t + 1 < 2 * nblk) load_step(t + 1);
+    lds_barrier();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s_t[(16 * wr + 4 * lq + v) * PT + wc + 16 * j + li] -= acc[j][v];
+    lds_barrier();
+    // X = inv * T, bf16x3
+    f32x4 acc2[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc2[j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      bf16x8 ah, al;
+      split_bf16(dv + 8 * kk, ah, al);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        float y[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) y[e] = s_t[(32 * kk + 8 * lq + e) * PT + wc + 16 * j + li];
+        bf16x8 bh, bl;
+        split_bf16(y, bh, bl);
+        acc2[j] = mfma_bf16(al, bh, acc2[j]);
+        acc2[j] = mfma_bf16(ah, bl, acc2[j]);
+        acc2[j] = mfma_bf16(ah, bh, acc2[j]);
+      }
+    }
+    if (t + 1 < 2 * nblk) load_dinv(t + 1);
+    lds_barrier();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s_t[(16 * wr + 4 * lq + v) * PT + wc + 16 * j + li] = acc2[j][v];
+    lds_barrier();
+    // write back: X_out coalesced, and the rows into s_y as bf16 octets
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (fr < nr && fc + CS * q < nc) xout_col[q][r0 + fr] = s_t[fr * PT + fc + CS * q];
+    for (int p = tid; p < (RB / 8) * CB; p += NT) {
+      const int o = p / CB, c = p % CB;
+      bf16x8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = bf16(s_t[(8 * o + e) * PT + c]);
+      *(bf16x8*)(s_y + (((r0 >> 3) + o) * CB + c) * 8) = v;
+    }
+    lds_barrier();
+  };
+  load_rhs_fwd(0);
+  load_dinv(0);
+  load_step(0);
+  for (int t = 0; t < nblk; ++t) step(t, std::true_type());
+  // the backward steps read rows of X_out the forward steps wrote: one full
+  // barrier (global memory included) between the two passes
+  __syncthreads();
+  for (int t = nblk; t < 2 * nblk; ++t) step(t, std::false_type());
+}
+
+// LU factors -> the bf16 copy trsm_lds_kernel reads, stored as MFMA A
+// fragments: element (i, k) (row i, column k of LU) at
+//   (((i/16) * (ldd/32) + k/32) * 64 + (i%16) + 16*((k%32)/8)) * 8 + k%8
+// so the 16 rows x 32 k of one fragment are 1 KB in lane order.  Rows up to
+// the next multiple of 16 and columns up to ldd are zero past m.  64 x 64
+// tiles through LDS.
+__global__ __launch_bounds__(kThreads) void transpose_bf16_kernel(const TransDesc* __restrict__ descs, int ndesc) {
+  constexpr int TB = 64;
+  __shared__ float s[TB * (TB + 1)];
+  const int64_t bid = blockIdx.x;
+  const TransDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int64_t t = bid - d.tile_begin;
+  const int i0 = int(t / d.tiles_k) * TB, k0 = int(t % d.tiles_k) * TB;
+  const HODLR_GLOBAL float* src = glob((const float*)d.src);
+  HODLR_GLOBAL bf16x8* dst = (HODLR_GLOBAL bf16x8*)glob((bf16*)d.dst);
+  const int m = d.m, mr = (m + 15) & ~15;
+  const int64_t nks = d.ldd >> 5;
+  const int tid = threadIdx.x, r = tid % TB, w = tid / TB;
+  float v[TB * TB / kThreads];
+#pragma unroll
+  for (int q = 0; q < TB * TB / kThreads; ++q) {
+    const int k = w + (kThreads / TB) * q;
+    v[q] = *(i0 + r < m && k0 + k < m ? src + (i0 + r) + int64_t(k0 + k) * d.lds : zero_page<float>());
+  }
+#pragma unroll
+  for (int q = 0; q < TB * TB / kThreads; ++q) s[(w + (kThreads / TB) * q) * (TB + 1) + r] = v[q];
+  __syncthreads();
+  // 8 fragments per tile (4 row groups x 2 k steps), one 16-byte octet per
+  // thread and pass: fragment f, lane l -> row 16(f%4) + l%16, k 32(f/4) + 8(l/16)
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const int f = tid / 64 + 4 * pass, l = tid % 64;
+    const int i = 16 * (f % 4) + (l % 16), kq = 32 * (f / 4) + 8 * (l / 16);
+    if (i0 + i < mr && k0 + kq < d.ldd) {
+      bf16x8 o;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) o[e] = bf16(s[(kq + e) * (TB + 1) + i]);
+      dst[((int64_t((i0 + i) >> 4) * nks + ((k0 + kq) >> 5)) * 64) + l] = o;
+    }
+  }
+}
+
 // ====================================================================== norms
 __global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restrict__ descs, int ndesc) {
   const int64_t bid = blockIdx.x;
@@ -983,6 +1259,34 @@ hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDe
       hipLaunchKernelGGL((trsm128_kernel<float, float, float, 128>), g, b, 0, s, d, n);
     else return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  if (mmax > 1024 || (variant != 2 && variant != 3) || (tin != F64 && tin != F32)) return hipErrorInvalidValue;
+  dim3 g(grid1(tiles)), b(kThreads);
+#define HODLR_TRSM_LDS(TI, MM, CB) hipLaunchKernelGGL((trsm_lds_kernel<TI, MM, CB>), g, dim3(CB * 8), 0, s, d, n)
+#define HODLR_TRSM_LDS2(MM, CB)                      \
+  do {                                              \
+    if (tin == F64) HODLR_TRSM_LDS(double, MM, CB); \
+    else HODLR_TRSM_LDS(float, MM, CB);             \
+  } while (0)
+  if (mmax <= 256) {
+    if (variant == 2) HODLR_TRSM_LDS2(256, 64);
+    else HODLR_TRSM_LDS2(256, 32);
+  } else {
+    if (variant == 2) HODLR_TRSM_LDS2(1024, 64);
+    else HODLR_TRSM_LDS2(1024, 32);
+  }
+#undef HODLR_TRSM_LDS2
+#undef HODLR_TRSM_LDS
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose_bf16(const TransDesc* d, int n, int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transpose_bf16_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
   return hipGetLastError();
 }
 

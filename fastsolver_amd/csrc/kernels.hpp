@@ -40,7 +40,14 @@ hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
 // variant 0: 64-row steps, 64-column panels; 1: 128-row steps, trsm_panel_cols(tlu)-column panels
 hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDesc* d, int n, int64_t tiles,
                        hipStream_t s);
-inline int trsm_panel_cols(DT tlu, int variant) { return variant == 0 ? kTrsmCB : (tlu == F64 ? 64 : 128); }
+// variants 2 / 3 (bf16 plans, TLU = F32, ut = BF16): the panel's solved rows kept in LDS as bf16,
+// 64 / 32 columns per workgroup, factor rows from the transposed bf16 copy (TrsmDesc::LUt);
+// `mmax` is the largest block of the batch (<= 1024)
+inline int trsm_panel_cols(DT tlu, int variant) {
+  return variant == 0 ? kTrsmCB : variant == 1 ? (tlu == F64 ? 64 : 128) : variant == 2 ? 64 : 32;
+}
+hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s);
+hipError_t launch_transpose_bf16(const TransDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s);
 

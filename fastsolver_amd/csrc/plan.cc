@@ -51,7 +51,7 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
                                 "solve_bcast"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
-enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN };
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS };
 
 struct Launch {
   Kind kind;
@@ -95,6 +95,8 @@ struct SysState {
   size_t mvx = 0;                  // hodlr_matvec: the caller's x staged here (N x c FP64, ld N)
   std::vector<size_t> lu_off;      // per leaf (elements)
   std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
+  size_t lut = 0;                  // bf16 plans: transposed bf16 copy of the leaf LU factors
+  std::vector<size_t> lut_off;     // per leaf (elements), ld = m rounded up to 32
   std::vector<size_t> z_off;       // per leaf (doubles)
   std::vector<size_t> vtd, vtu, wmv;   // per node (doubles): reduce outputs
   std::vector<int> leaf_lo, leaf_hi;   // per node: its leaves are leaf_id leaf_lo .. leaf_hi-1
@@ -425,30 +427,49 @@ struct Builder {
     if (ut != BF16) ut = tlu;
     std::vector<TrsmDesc> d;
     double flops = 0, bytes = 0;
+    int mmax = 0;
     for (auto& x : v) {
       if (x.m <= 0 || x.ncols <= 0) continue;
       flops += 2.0 * double(x.m) * x.m * x.ncols;
       bytes += double(x.m) * x.m * esize(tlu) + double(x.m) * x.ncols * (esize(tin) + esize(tout));
+      mmax = std::max(mmax, x.m);
       d.push_back(x);
     }
     if (d.empty()) return;
     Launch L;
     L.kind = K_TRSM; L.cat = cat; L.t[0] = tlu; L.t[1] = tin; L.t[2] = tout; L.t[3] = ut;
-    L.count = int(d.size()); L.flops = flops; L.bytes = bytes;
-    const bool both = ut != BF16;   // bf16 updates exist for the 64-row variant only
-    for (int var = 0; var < (both ? 2 : 1); ++var) {
+    L.count = int(d.size()); L.flops = flops; L.bytes = bytes; L.mmax = mmax;
+    // choices 0 / 1: 64-row / 128-row steps; bf16 plans: LDS-resident solved
+    // rows with 64 / 32 columns per workgroup (variants 2 / 3)
+    for (int cfg = 0; cfg < 2; ++cfg) {
+      const int var = ut == BF16 ? 2 + cfg : cfg;
       int64_t t = 0;
       for (auto& x : d) {
         x.tile_begin = t;
         t += cdiv(x.ncols, trsm_panel_cols(tlu, var));
       }
-      L.off_t[var] = blob.add(d);
-      L.tiles_t[var] = t;
+      L.off_t[cfg] = blob.add(d);
+      L.tiles_t[cfg] = t;
     }
-    L.tile = 0;   // 64-row steps: the faster variant on every measured workload
-    if (tunable && both) L.gid = int(ngid++);
+    L.tile = 0;   // the faster choice on every measured workload
+    if (tunable) L.gid = int(ngid++);
     L.off = L.off_t[L.tile];
     L.tiles = L.tiles_t[L.tile];
+    pr.l.push_back(L);
+  }
+
+  // transposed bf16 copies of LU factors (bf16 plans)
+  void transpose(Proc& pr, Cat cat, std::vector<TransDesc> v) {
+    int64_t t = 0;
+    double bytes = 0;
+    for (auto& x : v) {
+      x.tile_begin = t;
+      t += cdiv(x.m, 64) * x.tiles_k;
+      bytes += double(x.m) * x.m * 4 + double(x.m) * x.ldd * 2;
+    }
+    if (v.empty()) return;
+    Launch L;
+    L.kind = K_TRANS; L.cat = cat; L.off = blob.add(v); L.count = int(v.size()); L.tiles = t; L.bytes = bytes;
     pr.l.push_back(L);
   }
 
@@ -519,6 +540,16 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     }
     S.lu = L.take(lu_elems * we);
     S.ldinv = L.take(dinv_elems * we);
+    if (P->mt == BF16) {
+      size_t lut_elems = 0;
+      S.lut_off.resize(t.leaves.size());
+      for (size_t j = 0; j < t.leaves.size(); ++j) {
+        const int64_t m = t.nodes[t.leaves[j]].nrow;
+        S.lut_off[j] = lut_elems;
+        lut_elems += size_t(cdiv(m, 16) * 16) * size_t(cdiv(m, 32) * 32);
+      }
+      S.lut = L.take(lut_elems * 2);
+    }
     S.zbuf = L.take(std::max<size_t>(z_elems, 1) * 8);
     const size_t nn = t.nodes.size();
     S.leaf_lo.assign(nn, int(t.leaves.size()));
@@ -676,6 +707,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     std::vector<CopyDesc> kc;
     std::vector<LuDesc> lus;
     std::vector<TrsmDesc> tr, trs;
+    std::vector<TransDesc> tt;
     for (SysState& S : P->sys) {
       const Tree& t = S.tree;
       for (size_t j = 0; j < t.leaves.size(); ++j) {
@@ -700,6 +732,13 @@ void build_all(hodlr_plan_s* P, Builder& B) {
         lus.push_back(l);
         TrsmDesc d{};
         d.LU = l.A; d.ldlu = m; d.perm = l.perm; d.dinv = l.dinv;
+        if (P->mt == BF16) {
+          d.LUt = P->ws.p + S.lut + S.lut_off[j] * 2;
+          d.ldt = cdiv(m, 32) * 32;
+          TransDesc x{};
+          x.src = l.A; x.lds = m; x.dst = (void*)d.LUt; x.ldd = d.ldt; x.m = m; x.tiles_k = int(cdiv(d.ldt, 64));
+          tt.push_back(x);
+        }
         d.Xin = S.s.X + x.row_beg; d.ldin = S.s.ldx;
         d.Xout = P->ws.p + S.xws + size_t(x.row_beg) * we; d.ldout = S.ldw;
         d.m = m; d.ncols = int(t.leaf_nrhs(t.leaves[j]));
@@ -725,6 +764,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     }
     B.copy(P->leaf, CAT_LEAF_COPY, F64, wt, kc);
     B.lu(P->leaf, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, wt, lus, P->mt);
+    B.transpose(P->leaf, CAT_LU_MISC, tt);
     B.trsm(P->leaf, CAT_LEAF_TRSM, wt, F64, wt, tr, P->mt);
     B.trsm(P->solve_only_leaf, CAT_SOLVE_TRSM, wt, F64, wt, trs, P->mt);
   }
@@ -918,14 +958,16 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
     case K_TRSM: {
-      int var = L.tile;
+      int cfg = L.tile;
       if (L.gid >= 0) {
-        if (P->tuning) var = P->tune_cfg;
-        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) var = P->tile_choice[L.gid];
+        if (P->tuning) cfg = P->tune_cfg;
+        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) cfg = P->tile_choice[L.gid];
       }
-      return launch_trsm(L.t[0], L.t[1], L.t[2], L.t[3], var, (const TrsmDesc*)(P->descs.p + L.off_t[var]), L.count,
-                         L.tiles_t[var], s);
+      const TrsmDesc* td = (const TrsmDesc*)(P->descs.p + L.off_t[cfg]);
+      if (L.t[3] == BF16) return launch_trsm_lds(L.t[1], 2 + cfg, L.mmax, td, L.count, L.tiles_t[cfg], s);
+      return launch_trsm(L.t[0], L.t[1], L.t[2], L.t[3], cfg, td, L.count, L.tiles_t[cfg], s);
     }
+    case K_TRANS: return launch_transpose_bf16((const TransDesc*)d, L.count, L.tiles, s);
     case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s);
     case K_NORM_FIN: return launch_norm_finish((const NormDesc*)d, L.count, s);
   }
