@@ -61,11 +61,22 @@ template <> __device__ __forceinline__ bf16 to_math<bf16, float>(float x) { retu
 #define HODLR_GLOBAL __attribute__((address_space(1)))
 template <class T> __device__ __forceinline__ HODLR_GLOBAL T* glob(T* p) { return (HODLR_GLOBAL T*)p; }
 
-// Out-of-range lanes of a staged tile load from here: the value is already
-// zero, so no select (and no wait for the load) follows the load.
-static __device__ __attribute__((aligned(16))) double g_zero_page[2];
+// Out-of-range lanes of a staged tile load from here (32 zero bytes): the
+// value is already zero, so no select (and no wait for the load) follows the load.
+static __device__ __attribute__((aligned(16))) double g_zero_page[4];
 template <class T> __device__ __forceinline__ const HODLR_GLOBAL T* zero_page() {
   return (const HODLR_GLOBAL T*)g_zero_page;
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads stay in
+// flight across it (a full __syncthreads() waits for them).  Callers exchange
+// data between waves through LDS alone across it; a kernel whose waves read
+// global memory other waves of the workgroup wrote uses __syncthreads() after
+// those writes.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // ------------------------------------------------------------- tile engine
@@ -312,7 +323,7 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
   Stager<MT, TA, TB, BM, BN, BK, TRANS, NTB> st;
   st.load(A, lda, r0, mlim, B, ldb, c0, nlim, kb, ke);
   st.store(lds, lds + T::BOFF);
-  __syncthreads();
+  lds_barrier();
   int stage = 0;
   int k0 = kb;
   // steps whose successor is a full k step of an interior tile: branch-free loads
@@ -329,7 +340,7 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
       __builtin_amdgcn_sched_barrier(0);
       MT* nxt = lds + (stage ^ 1) * T::STAGE;
       st.store(nxt, nxt + T::BOFF);
-      __syncthreads();
+      lds_barrier();
       stage ^= 1;
     }
   }
@@ -347,7 +358,7 @@ __device__ __forceinline__ void gemm_loop(Tile<MT, BM, BN, BK>& t, MT* lds, cons
       MT* nxt = lds + (stage ^ 1) * T::STAGE;
       st.store(nxt, nxt + T::BOFF);
     }
-    __syncthreads();
+    lds_barrier();
     stage ^= 1;
   }
 }

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __res
     t.for_each([&](int r, int c, AE v) {
       if (c / CH == q) cs[(c - q * CH) * PC + r] = v;
     });
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int s = 0; s < SPQ; ++s) {
       const int e = threadIdx.x + s * kThreads;
@@ -101,7 +101,126 @@ __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __res
         out[gr + gc * ldo] = TC(x);
       }
     }
-    __syncthreads();
+    lds_barrier();
+  }
+}
+
+// ============================================================ row-panel GEMM
+// bf16 plans' broadcast d -= u eta (gemm.cc:417-478), C (m x n) = alpha A
+// (m x k) B (k x n) + beta C with k <= 128, fp32 storage, bf16 MFMA: one
+// workgroup per 64-row panel of C sweeps all n columns in 64-column chunks.
+// The panel's rows of A are read once and kept in LDS as MFMA A fragments;
+// B (small, L2-resident) goes from global memory straight into the B
+// fragments, 32 bytes per lane and k step (B's leading dimension is a
+// multiple of 8 and its rows past k are zero); C streams through once, read
+// and written coalesced, the next chunk's loads in flight while a chunk
+// computes.  HBM traffic: C read + write plus A once.
+// Wave w computes columns 16w .. 16w+15 of a chunk for all 64 rows.
+__global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
+  constexpr int BM = 64, BN = 64, NKS = 4, PC = BM + 1;   // k <= 128: four 32-k MFMA steps
+  constexpr int CPT = BM * BN / kThreads;                 // C elements per thread and chunk
+  __shared__ __attribute__((aligned(16))) bf16x8 s_a[4 * NKS * 64];   // [m block][k step][lane]
+  __shared__ float s_c[BN * PC];                                      // [col][row]
+
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+  const GemmDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int r0 = int(bid - d.tile_begin) * BM;
+  const int m = d.m, n = d.n, k = d.k;
+  const int nks = (k + 31) / 32;
+  const HODLR_GLOBAL float* A = glob((const float*)d.A);
+  const HODLR_GLOBAL float* B = glob((const float*)d.B);
+  const HODLR_GLOBAL float* Cin = glob((const float*)d.Cin);
+  HODLR_GLOBAL float* Cout = glob((float*)d.Cout);
+  const int64_t lda = d.lda, ldb = d.ldb, ldi = d.ldcin, ldo = d.ldcout;
+  const float alpha = float(d.alpha), beta = float(d.beta);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lq = lane >> 4;
+  const int fr = tid % BM, fc = __builtin_amdgcn_readfirstlane(tid / BM);   // coalesced: row fr, cols fc + 4q
+  const bool rok = r0 + fr < m;
+
+  // A panel (64 rows x k) -> LDS fragments: element (i, kk) at lane
+  // (i % 16) + 16 ((kk % 32) / 8), slot kk % 8, of fragment (i / 16, kk / 32)
+  {
+    bf16* sa = (bf16*)s_a;
+    for (int c0 = 0; c0 < NKS * 32; c0 += 32) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kk = c0 + fc + 4 * q;
+        v[q] = *(rok && kk < k ? A + (r0 + fr) + int64_t(kk) * lda : zero_page<float>());
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kk = c0 + fc + 4 * q;
+        sa[(((fr / 16) * NKS + kk / 32) * 64 + (fr % 16) + 16 * ((kk % 32) / 8)) * 8 + kk % 8] = bf16(v[q]);
+      }
+    }
+  }
+  // the chunk's B fragments: column 16w + li, k = 32 ks + 8 lq .. + 7
+  f32x4 bv[NKS][2];
+  auto load_b = [&](int c0) {
+    const int col = c0 + 16 * wave + li;
+    const HODLR_GLOBAL f32x4* bc = (const HODLR_GLOBAL f32x4*)(B + int64_t(col < n ? col : 0) * ldb);
+    const HODLR_GLOBAL f32x4* z = (const HODLR_GLOBAL f32x4*)zero_page<double>();
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int kk = 32 * ks + 8 * lq;
+      const bool ok = col < n && ks < nks && kk + 8 <= ldb;
+      bv[ks][0] = *(ok ? bc + kk / 4 : z);
+      bv[ks][1] = *(ok ? bc + kk / 4 + 1 : z + 1);
+    }
+  };
+  float cv[CPT];
+  auto load_c = [&](int c0) {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int col = c0 + fc + 4 * q;
+      cv[q] = *(rok && col < n ? Cin + (r0 + fr) + int64_t(col) * ldi : zero_page<float>());
+    }
+  };
+  load_b(0);
+  load_c(0);
+  lds_barrier();
+
+  for (int c0 = 0; c0 < n; c0 += BN) {
+    // product: acc[mb] = A(rows of block mb) x B(this wave's 16 columns)
+    f32x4 acc[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      if (ks < nks) {
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          b[e] = bf16(bv[ks][0][e]);
+          b[4 + e] = bf16(bv[ks][1][e]);
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+          acc[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a[(mb * NKS + ks) * 64 + lane], b, acc[mb], 0, 0, 0);
+      }
+    }
+    // accumulators -> LDS [col][row]; the next chunk's loads go out once this
+    // chunk's C values are taken
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s_c[(16 * wave + li) * PC + 16 * mb + 4 * lq + v] = acc[mb][v];
+    float cnow[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) cnow[q] = cv[q];
+    if (c0 + BN < n) {
+      load_b(c0 + BN);
+      load_c(c0 + BN);
+    }
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int col = c0 + fc + 4 * q;
+      if (rok && col < n) Cout[(r0 + fr) + int64_t(col) * ldo] = alpha * s_c[(fc + 4 * q) * PC + fr] + beta * cnow[q];
+    }
+    lds_barrier();
   }
 }
 
@@ -837,15 +956,6 @@ __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Workgroup barrier that orders LDS only: outstanding global loads stay in
-// flight across it (a full __syncthreads() waits for them).  The kernels that
-// use it exchange data between waves through LDS alone.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // bf16x3 product of fp32 operands: a*b ~ ah*bh + ah*bl + al*bh with the hi/lo
 // bf16 splits x = xh + xl (relative error ~2^-16, the small terms added first)
 __device__ __forceinline__ void split_bf16(const float* x, bf16x8& h, bf16x8& l) {
@@ -1171,6 +1281,13 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   }
 #undef HODLR_GEMM3
 #undef HODLR_GEMM
+  return hipGetLastError();
+}
+
+hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  if (ts != F32 || mt != BF16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(panel_gemm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
   return hipGetLastError();
 }
 

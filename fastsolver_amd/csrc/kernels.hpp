@@ -28,6 +28,11 @@ constexpr int64_t kNormChunk = 65536;
 // tile 0: 64 x 64 workgroup tiles, 1: 128 x 128 (roles 1-3 only)
 hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int tile, const GemmDesc* d, int n,
                        int64_t tiles, hipStream_t s);
+// row-panel GEMM (k <= 128, fp32 storage, bf16 MFMA): one workgroup per 64-row panel of C
+// sweeping all columns; B's leading dimension a multiple of 8 with zero rows past k;
+// descriptors tiled by 64-row panels
+constexpr int kPanelKMax = 128;
+hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s);  // double partials
 hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s);
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s);

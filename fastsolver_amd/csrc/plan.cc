@@ -59,10 +59,11 @@ struct Launch {
   DT t[4];
   bool trans = false;
   int role = 0;
-  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32 (default choice)
-  int gid = -1;          // GEMM launches with both tilings 0/1: index into the plan's tile choices
-  size_t off_t[3] = {0, 0, 0};
-  int64_t tiles_t[3] = {0, 0, 0};
+  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32, 3 = row panel
+  int gid = -1;          // tunable launches: index into the plan's choices
+  int alt[2] = {0, 1};   // tunable launches: the tile of choice 0 and of choice 1
+  size_t off_t[4] = {0, 0, 0, 0};
+  int64_t tiles_t[4] = {0, 0, 0, 0};
   size_t off = 0;        // byte offset of the first descriptor in the blob
   int count = 0;
   int64_t tiles = 0;
@@ -261,7 +262,26 @@ struct Builder {
     L.trans = trans; L.role = role; L.tile = tile;
     L.count = int(d.size());
     L.flops = flops; L.bytes = bytes;
-    for (int cfg = 0; cfg < 3; ++cfg) {
+    // broadcasts (role 2, every k <= 128, no split) also get the row-panel
+    // kernel, the default for them; the engine tiling stays the alternative
+    int kmax = 0;
+    for (auto& x : d) kmax = std::max(kmax, x.k);
+    bool panel = role == 2 && kmax <= kPanelKMax && ta == F32 && tb == F32 && tc == F32 && mt == BF16;
+    for (auto& x : d) panel = panel && x.ldb % 8 == 0 && x.ldb >= x.k;
+    for (int cfg = 0; cfg < 4; ++cfg) {
+      if (cfg == 3) {
+        if (!panel) continue;
+        int64_t tiles = 0;
+        for (auto& x : d) {
+          x.tiles_m = int(cdiv(x.m, 64));
+          x.tiles_n = 1;
+          x.tile_begin = tiles;
+          tiles += x.tiles_m;
+        }
+        L.off_t[3] = blob.add(d);
+        L.tiles_t[3] = tiles;
+        continue;
+      }
       if (narrow ? cfg != 2 : (cfg == 2 || (cfg == 1 && role == 0))) continue;
       const int TM = cfg == 1 ? 128 : kGemmBM, TN = cfg == 1 ? 128 : (cfg == 2 ? kGemmNarrowBN : kGemmBN);
       int64_t tiles = 0;
@@ -274,9 +294,16 @@ struct Builder {
       L.off_t[cfg] = blob.add(d);
       L.tiles_t[cfg] = tiles;
     }
-    if (role >= 1 && tunable && !narrow) L.gid = int(ngid++);
-    L.off = L.off_t[tile];
-    L.tiles = L.tiles_t[tile];
+    if (panel) {
+      L.tile = 3;
+      L.alt[0] = 3;
+      L.alt[1] = narrow ? 2 : 0;
+      if (tunable) L.gid = int(ngid++);
+    } else if (role >= 1 && tunable && !narrow) {
+      L.gid = int(ngid++);
+    }
+    L.off = L.off_t[L.tile];
+    L.tiles = L.tiles_t[L.tile];
     pr.l.push_back(L);
     sum(pr, cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat, sums);
   }
@@ -583,7 +610,7 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
       const int64_t nd = t.col_beg(x.depth + 1);
       S.sbuf[i] = L.take(size_t(r) * r * 8);
       S.eta[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
-      if (we != 8) S.etaf[i] = L.take(2 * size_t(r) * std::max<int64_t>(nd, c) * we);
+      if (we != 8) S.etaf[i] = L.take(2 * size_t(cdiv(r, 8) * 8) * std::max<int64_t>(nd, c) * we);
       S.spiv[i] = L.take(size_t(r) * 4);
       S.sperm[i] = L.take(size_t(r) * 4);
       S.sdinv[i] = L.take(size_t(cdiv(r, 64)) * 2 * 64 * 64 * 8);
@@ -847,21 +874,26 @@ void build_all(hodlr_plan_s* P, Builder& B) {
           // FP32 plan first rounds both etas to FP32 once, so the broadcast
           // tiles stream 4-byte operands
           const void* etas[2] = {eta0, vtd[0]};
+          // (leading dimension r rounded up to 8, rows past r zero: the plan's
+          // workspace is cleared at creation and those rows are never written)
+          const int rp = int(cdiv(r, 8) * 8);
+          int ldeta = r;
           if (wt != F64) {
             char* ef = P->ws.p + S.etaf[pi];
             for (int k = 0; k < 2; ++k) {
               CopyDesc cd{};
-              cd.src = etas[k]; cd.lds = r; cd.dst = ef + size_t(k) * r * nd * we; cd.ldd = r;
+              cd.src = etas[k]; cd.lds = r; cd.dst = ef + size_t(k) * rp * nd * we; cd.ldd = rp;
               cd.rows = r; cd.cols = nd; cd.alpha = 1.0;
               eround.push_back(cd);
               etas[k] = cd.dst;
             }
+            ldeta = rp;
           }
           for (int k = 0; k < 2; ++k) {
             const TreeNode& C = t.nodes[ch[k]];
             Builder::G g{};
             g.A = F + (size_t(C.row_beg) + size_t(ucol) * S.ldw) * we; g.lda = S.ldw;
-            g.B = etas[k]; g.ldb = r;
+            g.B = etas[k]; g.ldb = ldeta;
             g.Cout = X + size_t(C.row_beg) * we; g.Cin = X + size_t(C.row_beg) * we; g.ldcin = g.ldcout = S.ldw;
             g.m = int(C.nrow); g.n = nd; g.k = r; g.alpha = -1.0; g.beta = 1.0;
             bc.push_back(g);
@@ -945,11 +977,12 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_GEMM: {
       int tile = L.tile;
       if (L.gid >= 0) {
-        if (P->tuning) tile = P->tune_cfg;
-        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) tile = P->tile_choice[L.gid];
+        if (P->tuning) tile = L.alt[P->tune_cfg];
+        else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) tile = L.alt[P->tile_choice[L.gid]];
       }
-      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile,
-                         (const GemmDesc*)(P->descs.p + L.off_t[tile]), L.count, L.tiles_t[tile], s);
+      const GemmDesc* gd = (const GemmDesc*)(P->descs.p + L.off_t[tile]);
+      if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], gd, L.count, L.tiles_t[3], s);
+      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s);
     }
     case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, 1, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
@@ -1271,6 +1304,12 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
   size_t ninfo = L.take(size_t(std::max(P->nnodes, 1)) * 4);
   P->ws.bytes = L.top;
   if (hipMalloc((void**)&P->ws.p, P->ws.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
+  // a cleared workspace: padding rows that no kernel writes read as zero
+  if (hipMemsetAsync(P->ws.p, 0, P->ws.bytes, P->stream) != hipSuccess ||
+      hipStreamSynchronize(P->stream) != hipSuccess) {
+    free_plan(P.get());
+    return HODLR_ERR_HIP;
+  }
   P->leaf_info = (int32_t*)(P->ws.p + linfo);
   P->node_info = (int32_t*)(P->ws.p + ninfo);
   // scratch must exist before descriptors reference it: size it with a dry run
