@@ -68,6 +68,15 @@ template <class T> __device__ __forceinline__ const HODLR_GLOBAL T* zero_page() 
   return (const HODLR_GLOBAL T*)g_zero_page;
 }
 
+// Launches of a captured refinement sweep carry a gate: a device flag that the
+// convergence check clears once the residual meets the tolerance (or the sweep
+// budget is spent), so the sweeps left in the graph exit at their first
+// instruction.  A null gate always runs.  The flag is written by an earlier
+// kernel of the same stream, whose writes a later kernel sees.
+__device__ __forceinline__ bool gated_off(const int32_t* gate) {
+  return gate != nullptr && *(const HODLR_GLOBAL int32_t*)gate == 0;
+}
+
 // Workgroup barrier that orders LDS only: outstanding global loads stay in
 // flight across it (a full __syncthreads() waits for them).  Callers exchange
 // data between waves through LDS alone across it; a kernel whose waves read

@@ -99,4 +99,23 @@ struct NormDesc {
   int64_t tile_begin;
 };
 
+// Device-side state of the refinement loop of one plan (hodlr_solve).  The
+// host writes tol and maxiter before a solve; the check kernel keeps the rest.
+struct RefineCtl {
+  int32_t active;        // 1: the next sweep runs (the gate of its launches)
+  int32_t iter;          // sweeps done
+  int32_t maxiter;
+  int32_t pad0;
+  double tol;
+  double relres;         // after the last residual
+  double relres_first;   // after the first residual
+};
+
+// Residual and right-hand-side 2-norms of one system's c columns.
+struct CheckDesc {
+  const double* rnorm;
+  const double* bnorm;
+  int32_t c, pad0;
+};
+
 }  // namespace hodlr

@@ -25,7 +25,9 @@ namespace hodlr {
 // ROLE only separates the symbol names of the callers in profiles:
 // 0 node/matvec products, 1 reduce V^T X, 2 broadcast d -= u eta, 3 LU trailing update.
 template <int ROLE, class TA, class TB, class TC, class MT, bool TRANS, int BM, int BN>
-__global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc,
+                                                           const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   constexpr int BK = sizeof(MT) == 2 ? kGemmBKbf16
                    : BN == kGemmNarrowBN ? kGemmNarrowBK
                    : (BM * BN >= 128 * 128 ? kGemmBigBK : (sizeof(MT) == 8 ? kGemmBK : kGemmBK32));
@@ -116,7 +118,9 @@ __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __res
 // and written coalesced, the next chunk's loads in flight while a chunk
 // computes.  HBM traffic: C read + write plus A once.
 // Wave w computes columns 16w .. 16w+15 of a chunk for all 64 rows.
-__global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc,
+                                                              const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   constexpr int BM = 64, BN = 64, NKS = 4, PC = BM + 1;   // k <= 128: four 32-k MFMA steps
   constexpr int CPT = BM * BN / kThreads;                 // C elements per thread and chunk
   __shared__ __attribute__((aligned(16))) bf16x8 s_a[4 * NKS * 64];   // [m block][k step][lane]
@@ -225,7 +229,9 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
 }
 
 // split-K chunks are added in chunk order: deterministic, no atomics
-__global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict__ descs, int ndesc,
+                                                       const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int64_t bid = blockIdx.x;
   const SumDesc d = descs[find_desc(descs, ndesc, bid)];
   const int64_t base = (bid - d.tile_begin) * kSumTile;
@@ -246,7 +252,9 @@ __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict
 
 // ======================================================================= copy
 template <class TS, class TD>
-__global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restrict__ descs, int ndesc,
+                                                        const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int64_t bid = blockIdx.x;
   const CopyDesc d = descs[find_desc(descs, ndesc, bid)];
   const int64_t base = (bid - d.tile_begin) * kCopyTile;
@@ -688,7 +696,9 @@ __global__ __launch_bounds__(128) void lu_dinv_kernel(const LuDesc* __restrict__
 // diagonal block (MFMA on an LDS-resident 64 x 64 pair), so no step runs a
 // serial substitution.
 template <class TLU, class TIN, class TOUT, class UT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void trsm_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void trsm_kernel(
+    const TrsmDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   // MT: type of the diagonal-block products and of the stored factor; UT: math
   // type of the products with the solved rows (MT, or bf16 in the bf16 plans)
   typedef TLU MT;
@@ -825,7 +835,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
 // blocks and the 64 x 64 coupling block between them.  CB = 64 (fp64) or 128
 // (fp32) columns per workgroup.
 template <class TLU, class TIN, class TOUT, int CB>
-__global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void trsm128_kernel(const TrsmDesc* __restrict__ descs, int ndesc,
+                                                           const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   typedef TLU MT;
   constexpr int BK = kGemmBigBK, RB = 128, HB = kDB, PT = CB + 1, PI = HB + 1;
   typedef Tile<MT, RB, CB, BK> Big;
@@ -972,7 +984,8 @@ __device__ __forceinline__ void split_bf16(const float* x, bf16x8& h, bf16x8& l)
 // each other's latencies.
 template <class TIN, int MMAX, int CB>
 __global__ __launch_bounds__(CB * 8) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc) {
+void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   constexpr int NT = CB * 8;
   // PF: factor fragments in flight per wave, enough for a whole step of
   // the largest blocks; with the right-hand side and the inverse fragments
@@ -1207,7 +1220,9 @@ __global__ __launch_bounds__(kThreads) void transpose_bf16_kernel(const TransDes
 }
 
 // ====================================================================== norms
-__global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restrict__ descs, int ndesc,
+                                                        const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int64_t bid = blockIdx.x;
   const NormDesc d = descs[find_desc(descs, ndesc, bid)];
   const int64_t chunk = bid - d.tile_begin;
@@ -1225,7 +1240,8 @@ __global__ __launch_bounds__(kThreads) void norm_kernel(const NormDesc* __restri
   if (threadIdx.x == 0) d.partial[chunk] = red[0];
 }
 
-__global__ void norm_finish_kernel(const NormDesc* __restrict__ descs, int ndesc) {
+__global__ void norm_finish_kernel(const NormDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ndesc) return;
   const NormDesc d = descs[i];
@@ -1234,17 +1250,54 @@ __global__ void norm_finish_kernel(const NormDesc* __restrict__ descs, int ndesc
   *d.out = sqrt(s);
 }
 
+// Convergence check of the refinement loop, on the device, so the sweeps of a
+// captured solve need no host round trip: relres = the largest ||r_j|| / ||b_j||
+// over every right-hand side of every system (||r_j|| where ||b_j|| = 0, +inf
+// for NaN).  mode 0 follows the first residual (iter = 0, relres_first set);
+// mode 1 ends a sweep (iter += 1).  Then active = !(relres <= tol) && iter <
+// maxiter: the gate of the next sweep.  One workgroup.
+__global__ __launch_bounds__(kThreads) void refine_check_kernel(const CheckDesc* __restrict__ descs, int ndesc,
+                                                                RefineCtl* ctl, int mode,
+                                                                const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  double worst = 0.0;
+  for (int s = threadIdx.x; s < ndesc; s += kThreads) {
+    const CheckDesc d = descs[s];
+    for (int j = 0; j < d.c; ++j) {
+      const double rn = d.rnorm[j], bn = d.bnorm[j];
+      double v = bn > 0.0 ? rn / bn : rn;
+      if (!(v == v)) v = INFINITY;
+      worst = fmax(worst, v);
+    }
+  }
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = worst;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rr = red[0];
+    const int iter = mode == 0 ? 0 : ctl->iter + 1;
+    ctl->iter = iter;
+    ctl->relres = rr;
+    if (mode == 0) ctl->relres_first = rr;
+    ctl->active = (!(rr <= ctl->tol) && iter < ctl->maxiter) ? 1 : 0;
+  }
+}
+
 // ================================================================== launchers
 namespace {
 inline unsigned grid1(int64_t tiles) { return unsigned(tiles); }
 }
 
 hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int tile, const GemmDesc* d, int n,
-                       int64_t tiles, hipStream_t s) {
+                       int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
 #define HODLR_GEMM(R, A, B, C, M, T, BM, BN) \
-  hipLaunchKernelGGL((gemm_kernel<R, A, B, C, M, T, BM, BN>), g, b, 0, s, d, n)
+  hipLaunchKernelGGL((gemm_kernel<R, A, B, C, M, T, BM, BN>), g, b, 0, s, d, n, gate)
   // one of the three tilings: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32
 #define HODLR_GEMM3(R, A, B, C, M, T)                                  \
   do {                                                                 \
@@ -1284,26 +1337,27 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   return hipGetLastError();
 }
 
-hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
+                             const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   if (ts != F32 || mt != BF16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(panel_gemm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
+  hipLaunchKernelGGL(panel_gemm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
   return hipGetLastError();
 }
 
-hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sum_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
+  hipLaunchKernelGGL(sum_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
   return hipGetLastError();
 }
 
-hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
-  if (ts == F64 && td == F64) hipLaunchKernelGGL((copy_kernel<double, double>), g, b, 0, s, d, n);
-  else if (ts == F64 && td == F32) hipLaunchKernelGGL((copy_kernel<double, float>), g, b, 0, s, d, n);
-  else if (ts == F32 && td == F64) hipLaunchKernelGGL((copy_kernel<float, double>), g, b, 0, s, d, n);
-  else if (ts == F32 && td == F32) hipLaunchKernelGGL((copy_kernel<float, float>), g, b, 0, s, d, n);
+  if (ts == F64 && td == F64) hipLaunchKernelGGL((copy_kernel<double, double>), g, b, 0, s, d, n, gate);
+  else if (ts == F64 && td == F32) hipLaunchKernelGGL((copy_kernel<double, float>), g, b, 0, s, d, n, gate);
+  else if (ts == F32 && td == F64) hipLaunchKernelGGL((copy_kernel<float, double>), g, b, 0, s, d, n, gate);
+  else if (ts == F32 && td == F32) hipLaunchKernelGGL((copy_kernel<float, float>), g, b, 0, s, d, n, gate);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1351,39 +1405,40 @@ hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s) {
 }
 
 hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDesc* d, int n, int64_t tiles,
-                       hipStream_t s) {
+                       hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   dim3 g(grid1(tiles)), b(kThreads);
   if (ut == BF16) {
     if (variant != 0 || tlu != F32 || tout != F32) return hipErrorInvalidValue;
-    if (tin == F64) hipLaunchKernelGGL((trsm_kernel<float, double, float, bf16>), g, b, 0, s, d, n);
-    else if (tin == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float, bf16>), g, b, 0, s, d, n);
+    if (tin == F64) hipLaunchKernelGGL((trsm_kernel<float, double, float, bf16>), g, b, 0, s, d, n, gate);
+    else if (tin == F32) hipLaunchKernelGGL((trsm_kernel<float, float, float, bf16>), g, b, 0, s, d, n, gate);
     else return hipErrorInvalidValue;
   } else if (variant == 0) {
     if (tlu == F64 && tin == F64 && tout == F64)
-      hipLaunchKernelGGL((trsm_kernel<double, double, double, double>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm_kernel<double, double, double, double>), g, b, 0, s, d, n, gate);
     else if (tlu == F32 && tin == F64 && tout == F32)
-      hipLaunchKernelGGL((trsm_kernel<float, double, float, float>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm_kernel<float, double, float, float>), g, b, 0, s, d, n, gate);
     else if (tlu == F32 && tin == F32 && tout == F32)
-      hipLaunchKernelGGL((trsm_kernel<float, float, float, float>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm_kernel<float, float, float, float>), g, b, 0, s, d, n, gate);
     else return hipErrorInvalidValue;
   } else {
     if (tlu == F64 && tin == F64 && tout == F64)
-      hipLaunchKernelGGL((trsm128_kernel<double, double, double, 64>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm128_kernel<double, double, double, 64>), g, b, 0, s, d, n, gate);
     else if (tlu == F32 && tin == F64 && tout == F32)
-      hipLaunchKernelGGL((trsm128_kernel<float, double, float, 128>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm128_kernel<float, double, float, 128>), g, b, 0, s, d, n, gate);
     else if (tlu == F32 && tin == F32 && tout == F32)
-      hipLaunchKernelGGL((trsm128_kernel<float, float, float, 128>), g, b, 0, s, d, n);
+      hipLaunchKernelGGL((trsm128_kernel<float, float, float, 128>), g, b, 0, s, d, n, gate);
     else return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s,
+                           const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   if (mmax > 1024 || (variant != 2 && variant != 3) || (tin != F64 && tin != F32)) return hipErrorInvalidValue;
   dim3 g(grid1(tiles)), b(kThreads);
-#define HODLR_TRSM_LDS(TI, MM, CB) hipLaunchKernelGGL((trsm_lds_kernel<TI, MM, CB>), g, dim3(CB * 8), 0, s, d, n)
+#define HODLR_TRSM_LDS(TI, MM, CB) hipLaunchKernelGGL((trsm_lds_kernel<TI, MM, CB>), g, dim3(CB * 8), 0, s, d, n, gate)
 #define HODLR_TRSM_LDS2(MM, CB)                      \
   do {                                              \
     if (tin == F64) HODLR_TRSM_LDS(double, MM, CB); \
@@ -1407,15 +1462,22 @@ hipError_t launch_transpose_bf16(const TransDesc* d, int n, int64_t tiles, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(norm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
+  hipLaunchKernelGGL(norm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
   return hipGetLastError();
 }
 
-hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s) {
+hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s, const int32_t* gate) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(norm_finish_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d, n);
+  hipLaunchKernelGGL(norm_finish_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d, n, gate);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine_check(const CheckDesc* d, int n, RefineCtl* ctl, int mode, hipStream_t s,
+                               const int32_t* gate) {
+  if (n <= 0 || !ctl) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(kThreads), 0, s, d, n, ctl, mode, gate);
   return hipGetLastError();
 }
 

@@ -26,15 +26,19 @@ constexpr int64_t kNormChunk = 65536;
 //   (F64,F64,F64,F64) N and T;  (F64,F32,F64,F32) T;  (F32,F64,F32,F32) N;  (F32,F32,F32,F32) N;
 //   (F64,F32,F64,BF16) T;  (F32,F32,F32,BF16) N.
 // tile 0: 64 x 64 workgroup tiles, 1: 128 x 128 (roles 1-3 only)
+// `gate` (every launcher that takes one): a device flag, the launch does nothing while it reads 0
 hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int tile, const GemmDesc* d, int n,
-                       int64_t tiles, hipStream_t s);
+                       int64_t tiles, hipStream_t s, const int32_t* gate = nullptr);
 // row-panel GEMM (k <= 128, fp32 storage, bf16 MFMA): one workgroup per 64-row panel of C
 // sweeping all columns; B's leading dimension a multiple of 8 with zero rows past k;
 // descriptors tiled by 64-row panels
 constexpr int kPanelKMax = 128;
-hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s);
-hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s);  // double partials
-hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s);
+hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
+                             const int32_t* gate = nullptr);
+hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s,
+                      const int32_t* gate = nullptr);  // double partials
+hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s,
+                       const int32_t* gate = nullptr);
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s);
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
 // inverted 64 x 64 diagonal blocks b0 .. b0+nblk-1 of every factor
@@ -44,16 +48,20 @@ hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0
 // products with the solved rows, which run in `ut` (= TLU, or BF16 with TLU = F32, variant 0 only)
 // variant 0: 64-row steps, 64-column panels; 1: 128-row steps, trsm_panel_cols(tlu)-column panels
 hipError_t launch_trsm(DT tlu, DT tin, DT tout, DT ut, int variant, const TrsmDesc* d, int n, int64_t tiles,
-                       hipStream_t s);
+                       hipStream_t s, const int32_t* gate = nullptr);
 // variants 2 / 3 (bf16 plans, TLU = F32, ut = BF16): the panel's solved rows kept in LDS as bf16,
 // 64 / 32 columns per workgroup, factor rows from the transposed bf16 copy (TrsmDesc::LUt);
 // `mmax` is the largest block of the batch (<= 1024)
 inline int trsm_panel_cols(DT tlu, int variant) {
   return variant == 0 ? kTrsmCB : variant == 1 ? (tlu == F64 ? 64 : 128) : variant == 2 ? 64 : 32;
 }
-hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s);
+hipError_t launch_trsm_lds(DT tin, int variant, int mmax, const TrsmDesc* d, int n, int64_t tiles, hipStream_t s,
+                           const int32_t* gate = nullptr);
 hipError_t launch_transpose_bf16(const TransDesc* d, int n, int64_t tiles, hipStream_t s);
-hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s);
-hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s);
+hipError_t launch_norm(const NormDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate = nullptr);
+hipError_t launch_norm_finish(const NormDesc* d, int n, hipStream_t s, const int32_t* gate = nullptr);
+// refinement convergence check (one workgroup): mode 0 after the first residual, 1 after a sweep
+hipError_t launch_refine_check(const CheckDesc* d, int n, RefineCtl* ctl, int mode, hipStream_t s,
+                               const int32_t* gate = nullptr);
 
 }  // namespace hodlr

@@ -51,7 +51,8 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
                                 "solve_bcast"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
-enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS };
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS,
+            K_CHECK };
 
 struct Launch {
   Kind kind;
@@ -68,6 +69,8 @@ struct Launch {
   int count = 0;
   int64_t tiles = 0;
   int k0 = 0, rows_max = 0, mmax = 0;
+  int mode = 0;          // K_CHECK: 0 after the first residual, 1 after a sweep
+  bool gated = false;    // a refinement-sweep launch: runs only while the plan's RefineCtl::active is set
   double flops = 0, bytes = 0;   // algorithmic work of the launch (profiling)
 };
 
@@ -137,9 +140,13 @@ struct hodlr_plan_s {
   std::vector<Proc> reduce_s, node_s, bcast_s;    // solve-only (c columns)
   Proc solve_only_leaf;
   Proc factor;                                    // everything of the fused pass
-  // hip graph
-  bool use_graph = false;
-  hipGraphExec_t graph_exec = nullptr;
+  // device-driven refinement (hodlr_solve): `first` = residual of the factor
+  // pass's solution + convergence check; `sweep` = correction solve, x += y,
+  // residual, check, every launch gated by RefineCtl::active
+  Proc first, sweep;
+  size_t ctl = 0;                  // RefineCtl, byte offset in ws
+  RefineCtl ctl_host{};            // host copy (written before a solve, read after)
+  int sweep_hint = 0;              // sweeps the last hodlr_solve took
   // profiling: event pairs around the launches of the selected categories,
   // recorded asynchronously and harvested by hodlr_profile_read
   uint32_t prof_mask = 0;
@@ -148,11 +155,19 @@ struct hodlr_plan_s {
   double prof_flops[CAT_N] = {0}, prof_bytes[CAT_N] = {0};
   struct Pending { int cat; hipEvent_t a, b; double flops, bytes; };
   std::vector<Pending> pending;
-  // event pairs recorded as nodes of the captured graph: every replay records
-  // them again, so they are read after each replay, before the next one
-  bool capturing = false;
-  std::vector<Pending> graph_ev;
-  bool graph_ev_armed = false;
+  // hip graphs: the fused pass (hodlr_factor_solve); a whole solve (fused pass,
+  // first residual and `sweeps` gated sweeps); one sweep.  Event pairs
+  // recorded as nodes of a graph are recorded again at every replay, so they
+  // are read after each replay, before the next one.
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<Pending> ev;
+    bool armed = false;
+    int sweeps = 0;
+  };
+  bool use_graph = false;
+  Graph g_factor, g_solve, g_sweep;
+  std::vector<Pending>* cap_ev = nullptr;   // while capturing: the graph's event list
   std::vector<hipEvent_t> pool;
   // GEMM tilings: per tunable launch the chosen tile (0: 64 x 64, 1: 128 x 128)
   std::vector<int> tile_choice;
@@ -185,6 +200,7 @@ struct Builder {
   size_t scratch_need = 0;   // doubles
   size_t ngid = 0;           // GEMM launches with both tilings
   bool tunable = false;      // the plan's own launches (one-off batches are not tuned)
+  bool bad_gate = false;     // a refinement sweep holds a launch kind without a gate
 
   char* wsp(size_t off) { return P->ws.p + off; }
 
@@ -519,6 +535,21 @@ struct Builder {
     Launch F;
     F.kind = K_NORM_FIN; F.cat = CAT_NORM; F.off = off; F.count = int(v.size());
     pr.l.push_back(F);
+  }
+
+  // refinement convergence check over every system's residual and rhs norms
+  void check(Proc& pr, int mode) {
+    std::vector<CheckDesc> v;
+    for (SysState& S : P->sys) {
+      CheckDesc c{};
+      c.rnorm = (const double*)(P->ws.p + S.rnorm);
+      c.bnorm = (const double*)(P->ws.p + S.bnorm);
+      c.c = S.tree.nrhs;
+      v.push_back(c);
+    }
+    Launch L;
+    L.kind = K_CHECK; L.cat = CAT_NORM; L.off = blob.add(v); L.count = int(v.size()); L.mode = mode;
+    pr.l.push_back(L);
   }
 };
 
@@ -967,12 +998,38 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     }
     B.copy(P->solve_out, CAT_COPY, wt, F64, yo);
   }
+  // ---------------------------------------------------------------- device-driven refinement
+  // first: the residual of the fused pass's solution and its check; sweep:
+  // the correction solve on that residual (already in rbuf), x += y, the new
+  // residual and its check, every launch gated by the check before it
+  P->first = P->residual;
+  B.check(P->first, 0);
+  P->sweep = Proc();
+  P->sweep.append(P->solve_only_leaf);
+  for (int h = 1; h <= P->height; ++h) {
+    P->sweep.append(P->reduce_s[h]);
+    P->sweep.append(P->node_s[h]);
+    P->sweep.append(P->bcast_s[h]);
+  }
+  P->sweep.append(P->update);
+  P->sweep.append(P->residual);
+  B.check(P->sweep, 1);
+  for (Launch& L : P->sweep.l) {
+    L.gated = true;
+    // every kind a sweep holds takes a gate; a launch that ignored it would
+    // run in the sweeps the check has switched off
+    if (L.kind != K_GEMM && L.kind != K_SUM && L.kind != K_COPY && L.kind != K_TRSM && L.kind != K_NORM &&
+        L.kind != K_NORM_FIN && L.kind != K_CHECK)
+      B.bad_gate = true;
+  }
 }
 
 // ------------------------------------------------------------------ execution
 hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
   const void* d = P->descs.p + L.off;
   hipStream_t s = P->stream;
+  RefineCtl* ctl = (RefineCtl*)(P->ws.p + P->ctl);
+  const int32_t* gate = L.gated ? &ctl->active : nullptr;
   switch (L.kind) {
     case K_GEMM: {
       int tile = L.tile;
@@ -981,12 +1038,14 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
         else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) tile = L.alt[P->tile_choice[L.gid]];
       }
       const GemmDesc* gd = (const GemmDesc*)(P->descs.p + L.off_t[tile]);
-      if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], gd, L.count, L.tiles_t[3], s);
-      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s);
+      if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], gd, L.count, L.tiles_t[3], s, gate);
+      return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s,
+                         gate);
     }
     case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, 1, s);
-    case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s);
-    case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s);
+    case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s, gate);
+    case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
+    case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
@@ -997,12 +1056,12 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
         else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) cfg = P->tile_choice[L.gid];
       }
       const TrsmDesc* td = (const TrsmDesc*)(P->descs.p + L.off_t[cfg]);
-      if (L.t[3] == BF16) return launch_trsm_lds(L.t[1], 2 + cfg, L.mmax, td, L.count, L.tiles_t[cfg], s);
-      return launch_trsm(L.t[0], L.t[1], L.t[2], L.t[3], cfg, td, L.count, L.tiles_t[cfg], s);
+      if (L.t[3] == BF16) return launch_trsm_lds(L.t[1], 2 + cfg, L.mmax, td, L.count, L.tiles_t[cfg], s, gate);
+      return launch_trsm(L.t[0], L.t[1], L.t[2], L.t[3], cfg, td, L.count, L.tiles_t[cfg], s, gate);
     }
     case K_TRANS: return launch_transpose_bf16((const TransDesc*)d, L.count, L.tiles, s);
-    case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s);
-    case K_NORM_FIN: return launch_norm_finish((const NormDesc*)d, L.count, s);
+    case K_NORM: return launch_norm((const NormDesc*)d, L.count, L.tiles, s, gate);
+    case K_NORM_FIN: return launch_norm_finish((const NormDesc*)d, L.count, s, gate);
   }
   return hipErrorInvalidValue;
 }
@@ -1019,7 +1078,10 @@ hipEvent_t take_event(hodlr_plan_s* P) {
 }
 
 int harvest(hodlr_plan_s* P) {
-  if (P->pending.empty() && !P->graph_ev_armed) return HODLR_OK;
+  hodlr_plan_s::Graph* graphs[3] = {&P->g_factor, &P->g_solve, &P->g_sweep};
+  bool armed = false;
+  for (auto* g : graphs) armed = armed || g->armed;
+  if (P->pending.empty() && !armed) return HODLR_OK;
   if (hipStreamSynchronize(P->stream) != hipSuccess) return HODLR_ERR_HIP;
   auto take = [&](const hodlr_plan_s::Pending& q) {
     float ms = 0;
@@ -1036,24 +1098,61 @@ int harvest(hodlr_plan_s* P) {
     P->pool.push_back(q.b);
   }
   P->pending.clear();
-  if (P->graph_ev_armed) {
-    P->graph_ev_armed = false;
-    for (auto& q : P->graph_ev)
+  for (auto* g : graphs) {
+    if (!g->armed) continue;
+    g->armed = false;
+    for (auto& q : g->ev)
       if (!take(q)) return HODLR_ERR_HIP;
   }
   return HODLR_OK;
 }
 
-// Drops the captured graph (and the event pairs recorded in it).
-void drop_graph(hodlr_plan_s* P) {
+// Drops one captured graph (and the event pairs recorded in it).
+void drop_one(hodlr_plan_s* P, hodlr_plan_s::Graph& g) {
   (void)harvest(P);
-  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
-  P->graph_exec = nullptr;
-  for (auto& q : P->graph_ev) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  for (auto& q : g.ev) {
     P->pool.push_back(q.a);
     P->pool.push_back(q.b);
   }
-  P->graph_ev.clear();
+  g.ev.clear();
+  g.sweeps = 0;
+}
+
+// Drops every captured graph.
+void drop_graph(hodlr_plan_s* P) {
+  drop_one(P, P->g_factor);
+  drop_one(P, P->g_solve);
+  drop_one(P, P->g_sweep);
+}
+
+int run(hodlr_plan_s* P, const Proc& pr);
+
+// Captures body() on the plan's stream into g (nothing runs).
+template <class F> int capture(hodlr_plan_s* P, hodlr_plan_s::Graph& g, F&& body) {
+  drop_one(P, g);
+  hipGraph_t gr;
+  if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
+  P->cap_ev = &g.ev;
+  int rc = body();
+  P->cap_ev = nullptr;
+  if (hipStreamEndCapture(P->stream, &gr) != hipSuccess) return HODLR_ERR_HIP;
+  if (rc || hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    g.exec = nullptr;
+    return rc ? rc : HODLR_ERR_HIP;
+  }
+  (void)hipGraphDestroy(gr);
+  return HODLR_OK;
+}
+
+// Replays g; the event pairs of its previous replay are read first.
+int replay(hodlr_plan_s* P, hodlr_plan_s::Graph& g) {
+  if (g.armed && harvest(P)) return HODLR_ERR_HIP;
+  if (hipGraphLaunch(g.exec, P->stream) != hipSuccess) return HODLR_ERR_HIP;
+  g.armed = !g.ev.empty();
+  return HODLR_OK;
 }
 
 int run(hodlr_plan_s* P, const Proc& pr) {
@@ -1073,7 +1172,7 @@ int run(hodlr_plan_s* P, const Proc& pr) {
       if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
       if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
       if (hipEventRecord(b, P->stream) != hipSuccess) return HODLR_ERR_HIP;
-      (P->capturing ? P->graph_ev : P->pending).push_back({int(L.cat), a, b, L.flops, L.bytes});
+      (P->cap_ev ? *P->cap_ev : P->pending).push_back({int(L.cat), a, b, L.flops, L.bytes});
     } else if (dispatch(P, L) != hipSuccess) {
       return HODLR_ERR_HIP;
     }
@@ -1163,30 +1262,67 @@ int sync_stream(hodlr_plan_s* P) {
 
 int run_factor(hodlr_plan_s* P) {
   if (P->use_graph) {
-    // the events of the previous replay are read before they are recorded again
-    if (P->graph_ev_armed && harvest(P)) return HODLR_ERR_HIP;
-    if (!P->graph_exec) {
-      hipGraph_t g;
-      if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
-      P->capturing = true;
-      int rc = run(P, P->factor);
-      P->capturing = false;
-      hipError_t e = hipStreamEndCapture(P->stream, &g);
-      if (e != hipSuccess) return HODLR_ERR_HIP;
-      if (rc || hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        P->graph_exec = nullptr;
-        return HODLR_ERR_HIP;
-      }
-      (void)hipGraphDestroy(g);
+    if (!P->g_factor.exec) {
+      int rc = capture(P, P->g_factor, [&] { return run(P, P->factor); });
+      if (rc) return rc;
     }
-    if (hipGraphLaunch(P->graph_exec, P->stream) != hipSuccess) return HODLR_ERR_HIP;
-    P->graph_ev_armed = !P->graph_ev.empty();
+    if (replay(P, P->g_factor)) return HODLR_ERR_HIP;
   } else {
     int rc = run(P, P->factor);
     if (rc) return rc;
   }
   P->factored = true;
+  return HODLR_OK;
+}
+
+// The refinement loop of hodlr_solve, driven from the device: the fused pass,
+// the first residual and its check, then sweeps gated by RefineCtl::active,
+// which the check after each residual clears once max ||r_j|| / ||b_j|| <=
+// tol or maxiter sweeps are done.  With graphs on, one replay holds the pass
+// and as many sweeps as the previous solve took, so a solve of the same kind
+// of system is one graph launch and one host synchronisation; further sweeps
+// (if still active) replay a one-sweep graph.
+int run_solve(hodlr_plan_s* P, double tol, int maxiter) {
+  RefineCtl* dctl = (RefineCtl*)(P->ws.p + P->ctl);
+  P->ctl_host = RefineCtl{};
+  P->ctl_host.tol = tol;
+  P->ctl_host.maxiter = std::max(maxiter, 0);
+  if (hipMemcpyAsync(dctl, &P->ctl_host, sizeof(RefineCtl), hipMemcpyHostToDevice, P->stream) != hipSuccess)
+    return HODLR_ERR_HIP;
+  int rc = HODLR_OK;
+  if (P->use_graph) {
+    const int S = std::min(P->sweep_hint, P->ctl_host.maxiter);
+    if (!P->g_solve.exec || P->g_solve.sweeps != S) {
+      rc = capture(P, P->g_solve, [&] {
+        int r = run(P, P->factor);
+        if (!r) r = run(P, P->first);
+        for (int i = 0; i < S && !r; ++i) r = run(P, P->sweep);
+        return r;
+      });
+      if (rc) return rc;
+      P->g_solve.sweeps = S;
+    }
+    rc = replay(P, P->g_solve);
+  } else {
+    rc = run(P, P->factor);
+    if (!rc) rc = run(P, P->first);
+  }
+  if (rc) return rc;
+  P->factored = true;
+  for (;;) {
+    if (hipMemcpyAsync(&P->ctl_host, dctl, sizeof(RefineCtl), hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
+        sync_stream(P))
+      return HODLR_ERR_HIP;
+    if (!P->ctl_host.active) break;
+    if (P->use_graph) {
+      if (!P->g_sweep.exec && (rc = capture(P, P->g_sweep, [&] { return run(P, P->sweep); }))) return rc;
+      rc = replay(P, P->g_sweep);
+    } else {
+      rc = run(P, P->sweep);
+    }
+    if (rc) return rc;
+  }
+  P->sweep_hint = P->ctl_host.iter;
   return HODLR_OK;
 }
 
@@ -1302,6 +1438,7 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
   if (layout_plan(P.get(), L)) return HODLR_ERR_ARG;
   size_t linfo = L.take(size_t(P->nleaves) * 4);
   size_t ninfo = L.take(size_t(std::max(P->nnodes, 1)) * 4);
+  P->ctl = L.take(sizeof(RefineCtl));
   P->ws.bytes = L.top;
   if (hipMalloc((void**)&P->ws.p, P->ws.bytes) != hipSuccess) { free_plan(P.get()); return HODLR_ERR_NOMEM; }
   // a cleared workspace: padding rows that no kernel writes read as zero
@@ -1328,6 +1465,7 @@ int hodlr_plan_create(hodlr_plan_t* plan, const hodlr_system_t* systems, int32_t
   P->leaf = Proc(); P->extract = Proc(); P->residual = Proc(); P->update = Proc(); P->solve_only_leaf = Proc();
   P->matvec = Proc(); P->solve_out = Proc();
   build_all(P.get(), B);
+  if (B.bad_gate) { free_plan(P.get()); return HODLR_ERR_UNSUPPORTED; }
   P->tile_choice.assign(B.ngid, -1);
   // the fused pass
   P->factor = Proc();
@@ -1643,26 +1781,12 @@ int hodlr_residual(hodlr_plan_t P, double* relres, double* relres_max) {
 int hodlr_solve(hodlr_plan_t P, double tol, int32_t maxiter, hodlr_solve_report_t* rep) {
   if (!P) return HODLR_ERR_ARG;
   hodlr_solve_report_t R{};
-  int rc = run_factor(P);
+  int rc = run_solve(P, tol, maxiter);
   if (rc) return rc;
-  double rr = 0;
-  rc = run(P, P->residual);
-  if (!rc) rc = read_relres(P, nullptr, &rr);
-  if (rc) return rc;
-  R.relres_first = rr;
-  int it = 0;
-  while (!(rr <= tol) && it < maxiter) {
-    // the residual already sits in the plan's right-hand-side buffer
-    rc = solve_only_internal(P);
-    if (!rc) rc = run(P, P->update);
-    if (!rc) rc = run(P, P->residual);
-    if (!rc) rc = read_relres(P, nullptr, &rr);
-    if (rc) return rc;
-    ++it;
-  }
-  R.iterations = it;
-  R.relres = rr;
-  R.converged = rr <= tol;
+  R.relres_first = P->ctl_host.relres_first;
+  R.iterations = P->ctl_host.iter;
+  R.relres = P->ctl_host.relres;
+  R.converged = R.relres <= tol;
   if (rep) *rep = R;
   return R.converged ? HODLR_OK : HODLR_ERR_NOTCONVERGED;
 }

@@ -131,7 +131,11 @@ int hodlr_matvec(hodlr_plan_t plan, const double* const* x, const int64_t* ldx, 
 int hodlr_residual(hodlr_plan_t plan, double* relres, double* relres_max);
 
 /* Full solve: factor + solve in the plan's precision, then FP64 iterative
- * refinement until max relative residual <= tol or maxiter sweeps. */
+ * refinement until max relative residual <= tol or maxiter sweeps.  The
+ * convergence check runs on the device and gates the sweeps; with
+ * hodlr_use_graph on, the pass, the first residual and as many sweeps as the
+ * previous call took are one graph replay (one host synchronisation per
+ * solve when the count repeats). */
 int hodlr_solve(hodlr_plan_t plan, double tol, int32_t maxiter, hodlr_solve_report_t* report);
 
 /* First zero pivot (1-based, 0 = none) of every leaf block (host array of
