@@ -1078,8 +1078,12 @@ void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc, const int32_
     constexpr bool FWD = decltype(fwd)::value;
     const int b = blk(t), r0 = b * RB, nr = min(RB, m - r0);
     const int kb = FWD ? 0 : r0 + RB, ke = FWD ? r0 : kend;
-    const int nk = ke > kb ? (ke - kb) >> 5 : 0;
     const bool rowok = r0 + 16 * wr < m;
+    // a row group past the block's last row has no product: load_step left
+    // its fragment registers unloaded (stale, or never written: possibly
+    // NaN bit patterns, which the zero columns of the inverse would not
+    // cancel), so its k loop is empty and its rows stay 0 - 0
+    const int nk = rowok && ke > kb ? (ke - kb) >> 5 : 0;
     const HODLR_GLOBAL bf16x8* arow = frag(rowok ? (r0 >> 4) + wr : 0, kb >> 5);
     // update: acc = factor block row x solved rows (bf16 MFMA, fp32 sums);
     // fragments beyond PF (only in the last steps of the largest blocks) are

@@ -1,8 +1,9 @@
 """The batched-uneven workload family at full size: 512 independent HODLR
 systems, N uniform in [4096, 65536] in steps of 256, leaves <= 256, per-level
 ranks in [8, 64], 1..16 right-hand sides, all in ONE plan (one launch per phase
-per height).  FP32 factor + FP64 refinement and plain FP64, checked by residual
-on every system and against the oracle on a sample of 16 systems (every 32nd)."""
+per height).  FP32 and BF16 factors + FP64 refinement and plain FP64, checked
+by residual on every system and against the oracle on a sample of 16 systems
+(every 32nd)."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,16 @@ def test_uneven_batch_512_systems():
     for i, xo in checked.items():
         assert rel(dss[i].solution(), xo) <= TOL, (i, specs[i])
     p32.close()
+    p16 = hodlr.Plan(dss, hodlr.PREC_BF16)
+    rep = p16.solve(tol=TOL, maxiter=10)
+    print(f"  bf16+refine: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}", flush=True)
+    assert rep.converged and rep.relres <= TOL
+    assert 1e-6 < rep.relres_first < 1e-2
+    _, per = p16.residual()
+    assert np.all(per <= TOL)
+    for i, xo in checked.items():
+        assert rel(dss[i].solution(), xo) <= TOL, (i, specs[i])
+    p16.close()
     p64 = hodlr.Plan(dss, hodlr.PREC_FP64)
     p64.factor_solve()
     rc, _, _ = p64.status()
@@ -71,3 +82,24 @@ def test_uneven_batch_after_b_changes():
     for s, h, d in zip(specs, hosts, dss):
         xo = oracle_solve(s, *h)[:, : s.nrhs]
         assert rel(d.solution(), xo) <= TOL, s
+
+
+@pytest.mark.parametrize("prec", [hodlr.PREC_FP32, hodlr.PREC_BF16])
+def test_leaf_sizes_with_partial_row_groups(prec):
+    """Leaves of 129..256 rows whose last 64-row block ends inside a 16-row
+    group or leaves whole 16-row groups empty (the BF16 TRSM's waves past a
+    leaf's last row must contribute nothing), in one plan with the oracle on
+    every system."""
+    sizes = [129, 161, 174, 193, 205, 207, 215, 220, 223, 239, 241, 250, 256]
+    specs = [gen.SystemSpec(4 * m, 256, 1 + (i % 5), [24 + i, 9 + i], seed=1245667 + i, d_scale=1 / 16)
+             for i, m in enumerate(sizes)]
+    hosts = [host_system(s) for s in specs]
+    dss = [hodlr.DeviceSystem.from_host(s, *h) for s, h in zip(specs, hosts)]
+    plan = hodlr.Plan(dss, prec)
+    rep = plan.solve(tol=TOL, maxiter=10)
+    print(f"  prec {prec}: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}", flush=True)
+    assert rep.converged and rep.relres <= TOL
+    for s, h, d in zip(specs, hosts, dss):
+        assert gen.tree_info(s)[0] == 4
+        xo = oracle_solve(s, *h)[:, : s.nrhs]
+        assert rel(d.solution(), xo) <= TOL, s.n
