@@ -99,6 +99,24 @@ struct NormDesc {
   int64_t tile_begin;
 };
 
+// FP64 streaming product with few columns (the residual and matvec sweeps):
+//   C (m x n) = alpha * (op(A1) B1 + A2 B2) + beta * Cin
+// op(A1) = A1 (m x k1) or A1^T (A1: k1 x m); A2 (m x k2, never transposed)
+// may be absent (k2 = 0).  One workgroup per 256 rows x 16 columns of C.
+struct StreamDesc {
+  const double* A1;
+  const double* B1;
+  const double* A2;
+  const double* B2;
+  const double* Cin;
+  double* Cout;
+  int64_t lda1, ldb1, lda2, ldb2, ldcin, ldcout;
+  int32_t m, n, k1, k2;
+  int32_t tiles_m, tiles_n;
+  double alpha, beta;
+  int64_t tile_begin;
+};
+
 // Device-side state of the refinement loop of one plan (hodlr_solve).  The
 // host writes tol and maxiter before a solve; the check kernel keeps the rest.
 struct RefineCtl {

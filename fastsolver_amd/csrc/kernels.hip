@@ -228,6 +228,160 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
   }
 }
 
+// ============================================================ FP64 streaming
+// C = alpha (op(A1) B1 + A2 B2) + beta Cin for products with few columns
+// (16 per tile): the residual r = b - (blockdiag(K) x + U Z) as one
+// two-segment product per leaf row block, and V^T x of the HODLR matvec
+// (src/solver/gemm.cc:315-384 and :417-478 on c columns, in FP64).  At c <= 16
+// these are HBM-bound (4 flop per byte of A, under the FP64 ridge of ~10), so
+// every wave owns 64 rows of C and the tile's columns and streams its rows of
+// A once, straight from global memory into FP64 MFMA operand registers: no
+// LDS and no barrier in the k loop, the next 16-k step's loads in flight
+// while a step computes.  In the e-th MFMA of a step, the k slot q of lane
+// (p, q) = (l % 16, l / 16) carries k0 + 4q + e, so a lane's B values are 4
+// consecutive rows of one column:
+//   NN: a(b, e) = A(r0 + 16b + p, k0 + 4q + e)   (16 rows of a column per 16 lanes)
+//   TN: a(b, e) = A(k0 + 4q + e, r0 + 16b + p)   (row i of C = column i of A)
+//   B:  b(c, e) = B(k0 + 4q + e, 16c + p)
+// Accumulator entry v of lane (p, q), block (b, c): C row 16b + q + 4v,
+// column 16c + p.  The tile goes out through LDS, coalesced.
+template <bool TRANS, int NCB>
+__device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR_GLOBAL double* A, int64_t lda,
+                                               const HODLR_GLOBAL double* B, int64_t ldb, int k, int r0, int m,
+                                               int c0, int n) {
+  constexpr int NB = 4, KS = 16;
+  if (k <= 0) return;
+  const int lane = threadIdx.x & 63, p = lane & 15, q = lane >> 4;
+  // Rows past m and columns past n read the last row / column instead (their
+  // results are never written), so the loads of a full k step need no
+  // predicate.  Each pointer already carries the lane's 4q shift along k.
+  const HODLR_GLOBAL double* arow[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int row = min(r0 + 16 * b + p, m - 1);
+    arow[b] = TRANS ? A + int64_t(row) * lda + 4 * q : A + row + int64_t(4 * q) * lda;
+  }
+  const HODLR_GLOBAL double* bcol[NCB];
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) bcol[c] = B + int64_t(min(c0 + 16 * c + p, n - 1)) * ldb + 4 * q;
+  const HODLR_GLOBAL double* zero = zero_page<double>();
+  // a full k step: no predicate, no clamp
+  auto load = [&](double (&a)[NB][4], double (&bb)[NCB][4], int k0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t off = TRANS ? int64_t(k0 + e) : int64_t(k0 + e) * lda;   // wave-uniform
+#pragma unroll
+      for (int b = 0; b < NB; ++b) a[b][e] = arow[b][off];
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) bb[c][e] = bcol[c][k0 + e];
+    }
+  };
+  // the last, partial k step: A at k >= k reads column (row) k - 1, B is 0 there
+  auto load_tail = [&](double (&a)[NB][4], double (&bb)[NCB][4], int k0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = k0 + 4 * q + e;
+      const bool kok = kk < k;
+      const int dk = (kok ? kk : k - 1) - 4 * q;
+      const int64_t off = TRANS ? int64_t(dk) : int64_t(dk) * lda;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) a[b][e] = arow[b][off];
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) bb[c][e] = *(kok ? bcol[c] + (k0 + e) : zero);
+    }
+  };
+  auto compute = [&](const double (&a)[NB][4], const double (&bb)[NCB][4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) acc[b][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b][e], bb[c][e], acc[b][c], 0, 0, 0);
+  };
+  // Full steps two at a time with the next step's loads issued before the
+  // current step's products: the loop body is straight-line, so the wait
+  // before each product lets the other set's loads stay in flight.
+  double a0[NB][4], b0[NCB][4], a1[NB][4], b1[NCB][4];
+  const int nfull = k / KS;
+  if (nfull > 0) {
+    load(a0, b0, 0);
+    int s = 0;
+    for (; s + 2 < nfull; s += 2) {
+      load(a1, b1, (s + 1) * KS);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(a0, b0, (s + 2) * KS);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (s + 1 < nfull) {
+      load(a1, b1, (s + 1) * KS);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a0, b0);
+      compute(a1, b1);
+    } else {
+      compute(a0, b0);
+    }
+  }
+  if (nfull * KS < k) {
+    load_tail(a0, b0, nfull * KS);
+    compute(a0, b0);
+  }
+}
+
+template <bool TRANS, int NCB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void stream_kernel(
+    const StreamDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  constexpr int NB = 4, WR = 64, PC = WR + 1, NC = 16 * NCB;
+  __shared__ double s_c[4 * 16 * PC];   // per wave: one 16-column block of its tile
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+  const StreamDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int64_t local = bid - d.tile_begin;
+  const int tm = int(local % d.tiles_m), tn = int(local / d.tiles_m);
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = lane & 15, q = lane >> 4;
+  const int r0 = tm * (4 * WR) + wave * WR, c0 = tn * NC;
+  const int m = d.m, n = d.n;
+  f64x4 acc[NB][NCB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) acc[b][c] = f64x4{0, 0, 0, 0};
+  if (r0 < m) {
+    stream_segment<TRANS, NCB>(acc, glob(d.A1), d.lda1, glob(d.B1), d.ldb1, d.k1, r0, m, c0, n);
+    if (d.k2 > 0) stream_segment<false, NCB>(acc, glob(d.A2), d.lda2, glob(d.B2), d.ldb2, d.k2, r0, m, c0, n);
+  }
+  // epilogue: each 16-column block of the wave's 64-row tile through the
+  // wave's LDS slice, then one row per lane, column by column (coalesced)
+  double* sc = s_c + wave * 16 * PC;
+  const int row = r0 + lane;
+  const double alpha = d.alpha, beta = d.beta;
+  const HODLR_GLOBAL double* cin = glob(d.Cin);
+  HODLR_GLOBAL double* cout = glob(d.Cout);
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+    const int cb = c0 + 16 * c;
+    double old[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      old[j] = (beta != 0.0 && row < m && cb + j < n) ? cin[row + int64_t(cb + j) * d.ldcin] : 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sc[p * PC + 16 * b + q + 4 * v] = acc[b][c][v];
+    __syncthreads();
+    if (row < m) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (cb + j < n) cout[row + int64_t(cb + j) * d.ldcout] = alpha * sc[j * PC + lane] + beta * old[j];
+    }
+    __syncthreads();
+  }
+}
+
 // split-K chunks are added in chunk order: deterministic, no atomics
 __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict__ descs, int ndesc,
                                                        const int32_t* __restrict__ gate) {
@@ -1050,9 +1204,14 @@ void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc, const int32_
     const int nk = ke > kb ? (ke - kb) >> 5 : 0;
     const bool ok = r0 + 16 * wr < m;
     const HODLR_GLOBAL bf16x8* p = frag(ok ? (r0 >> 4) + wr : 0, kb >> 5);
+    // a row group past the block's last row gets zero fragments: its k loop
+    // then adds nothing (registers left unloaded could hold NaN bit
+    // patterns, which the zero columns of the inverse would not cancel)
 #pragma unroll
-    for (int u = 0; u < PF; ++u)
+    for (int u = 0; u < PF; ++u) {
       if (ok && u < nk) ar[u] = p[64 * u];
+      else if (!ok) ar[u] = bf16x8{};
+    }
   };
   auto load_dinv = [&](int t) {
     const HODLR_GLOBAL float* inv = DI + int64_t(blk(t)) * 2 * RB * RB + (t < nblk ? 0 : RB * RB) + wrow;
@@ -1078,12 +1237,8 @@ void trsm_lds_kernel(const TrsmDesc* __restrict__ descs, int ndesc, const int32_
     constexpr bool FWD = decltype(fwd)::value;
     const int b = blk(t), r0 = b * RB, nr = min(RB, m - r0);
     const int kb = FWD ? 0 : r0 + RB, ke = FWD ? r0 : kend;
+    const int nk = ke > kb ? (ke - kb) >> 5 : 0;
     const bool rowok = r0 + 16 * wr < m;
-    // a row group past the block's last row has no product: load_step left
-    // its fragment registers unloaded (stale, or never written: possibly
-    // NaN bit patterns, which the zero columns of the inverse would not
-    // cancel), so its k loop is empty and its rows stay 0 - 0
-    const int nk = rowok && ke > kb ? (ke - kb) >> 5 : 0;
     const HODLR_GLOBAL bf16x8* arow = frag(rowok ? (r0 >> 4) + wr : 0, kb >> 5);
     // update: acc = factor block row x solved rows (bf16 MFMA, fp32 sums);
     // fragments beyond PF (only in the last steps of the largest blocks) are
@@ -1346,6 +1501,16 @@ hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t til
   if (n <= 0 || tiles <= 0) return hipSuccess;
   if (ts != F32 || mt != BF16) return hipErrorInvalidValue;
   hipLaunchKernelGGL(panel_gemm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream(bool trans, int ncols_max, const StreamDesc* d, int n, int64_t tiles, hipStream_t s,
+                         const int32_t* gate) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  if (ncols_max > kStreamCols) return hipErrorInvalidValue;
+  dim3 g(grid1(tiles)), b(kThreads);
+  if (trans) hipLaunchKernelGGL((stream_kernel<true, kStreamCols / 16>), g, b, 0, s, d, n, gate);
+  else hipLaunchKernelGGL((stream_kernel<false, kStreamCols / 16>), g, b, 0, s, d, n, gate);
   return hipGetLastError();
 }
 

@@ -35,6 +35,11 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int t
 constexpr int kPanelKMax = 128;
 hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
                              const int32_t* gate = nullptr);
+// FP64 streaming products (StreamDesc), tiles of kStreamRows rows x kStreamCols columns;
+// `ncols_max` = the widest product of the batch (at most kStreamCols)
+constexpr int kStreamRows = 256, kStreamCols = 16;
+hipError_t launch_stream(bool trans, int ncols_max, const StreamDesc* d, int n, int64_t tiles, hipStream_t s,
+                         const int32_t* gate = nullptr);
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s,
                       const int32_t* gate = nullptr);  // double partials
 hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s,

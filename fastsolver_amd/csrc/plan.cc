@@ -43,16 +43,19 @@ enum Cat {
   CAT_LEAF_COPY = 0, CAT_LU_PANEL, CAT_LU_ROW, CAT_LU_UPDATE, CAT_LU_MISC, CAT_LEAF_TRSM,
   CAT_REDUCE, CAT_REDUCE_SUM, CAT_NODE_GEMM, CAT_NODE_LU, CAT_NODE_TRSM, CAT_BCAST,
   CAT_COPY, CAT_MATVEC, CAT_NORM, CAT_OTHER, CAT_SOLVE_TRSM, CAT_SOLVE_REDUCE, CAT_SOLVE_NODE, CAT_SOLVE_BCAST,
+  CAT_MATVEC_VTX, CAT_MATVEC_ROWS,
   CAT_N
 };
+// matvec: the sums and gathers of the HODLR product; matvec_vtx: its V^T x
+// products; matvec_rows: its blockdiag(K) x + U Z products
 const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_update", "lu_misc", "leaf_trsm",
                                 "reduce_gemm", "reduce_sum", "node_gemm", "node_lu", "node_trsm", "bcast_gemm",
                                 "copy", "matvec", "norm", "other", "solve_trsm", "solve_reduce", "solve_node",
-                                "solve_bcast"};
+                                "solve_bcast", "matvec_vtx", "matvec_rows"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
 enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS,
-            K_CHECK };
+            K_CHECK, K_STREAM };
 
 struct Launch {
   Kind kind;
@@ -70,6 +73,7 @@ struct Launch {
   int64_t tiles = 0;
   int k0 = 0, rows_max = 0, mmax = 0;
   int mode = 0;          // K_CHECK: 0 after the first residual, 1 after a sweep
+  int nmax = 0;          // K_STREAM: the widest product of the batch (columns)
   bool gated = false;    // a refinement-sweep launch: runs only while the plan's RefineCtl::active is set
   double flops = 0, bytes = 0;   // algorithmic work of the launch (profiling)
 };
@@ -156,18 +160,15 @@ struct hodlr_plan_s {
   struct Pending { int cat; hipEvent_t a, b; double flops, bytes; };
   std::vector<Pending> pending;
   // hip graphs: the fused pass (hodlr_factor_solve); a whole solve (fused pass,
-  // first residual and `sweeps` gated sweeps); one sweep.  Event pairs
-  // recorded as nodes of a graph are recorded again at every replay, so they
-  // are read after each replay, before the next one.
+  // first residual and `sweeps` gated sweeps); one sweep.  Not used while a
+  // profile mask is set.
   struct Graph {
     hipGraphExec_t exec = nullptr;
-    std::vector<Pending> ev;
-    bool armed = false;
     int sweeps = 0;
   };
   bool use_graph = false;
   Graph g_factor, g_solve, g_sweep;
-  std::vector<Pending>* cap_ev = nullptr;   // while capturing: the graph's event list
+  bool graphs() const { return use_graph && prof_mask == 0; }
   std::vector<hipEvent_t> pool;
   // GEMM tilings: per tunable launch the chosen tile (0: 64 x 64, 1: 128 x 128)
   std::vector<int> tile_choice;
@@ -537,6 +538,62 @@ struct Builder {
     pr.l.push_back(F);
   }
 
+  // FP64 products with few columns (the residual and matvec sweeps), given as
+  // streaming descriptors: the streaming kernel while every product is at
+  // most kStreamCols wide (HBM-bound: A read once, no LDS; measured 4.3 vs
+  // 4.5 ms per matvec of 64 uneven systems with 1-16 rhs), else the tile
+  // engine's 64 x 32 LDS tiles (at 32 columns the FP64 MFMA rate binds and
+  // the engine measured faster: 7.4 vs 8.4 ms per matvec of N = 2^20, c = 32)
+  void few_cols(Proc& pr, Cat cat, bool trans, const std::vector<StreamDesc>& v) {
+    int nmax = 0;
+    for (auto& x : v) nmax = std::max(nmax, x.n);
+    if (nmax <= kStreamCols) {
+      stream(pr, cat, trans, v);
+      return;
+    }
+    std::vector<G> g1, g2;   // op(A1) B1 (+ beta Cin), then += A2 B2 in a second launch
+    for (auto& x : v) {
+      G a{};
+      a.A = x.A1; a.lda = x.lda1; a.B = x.B1; a.ldb = x.ldb1; a.Cin = x.Cin; a.ldcin = x.ldcin;
+      a.Cout = x.Cout; a.ldcout = x.ldcout; a.m = x.m; a.n = x.n; a.k = x.k1; a.alpha = x.alpha; a.beta = x.beta;
+      g1.push_back(a);
+      if (x.k2 > 0) {
+        G b = a;
+        b.A = x.A2; b.lda = x.lda2; b.B = x.B2; b.ldb = x.ldb2; b.Cin = x.Cout; b.ldcin = x.ldcout;
+        b.k = x.k2; b.beta = 1.0;
+        g2.push_back(b);
+      }
+    }
+    gemm(pr, cat, F64, F64, F64, F64, trans, trans ? 1 : 0, g1);
+    gemm(pr, cat, F64, F64, F64, F64, false, 0, g2);
+  }
+
+  // FP64 streaming products, tiles of kStreamRows rows x kStreamCols columns
+  void stream(Proc& pr, Cat cat, bool trans, std::vector<StreamDesc> v) {
+    int nmax = 0;
+    for (auto& x : v) nmax = std::max(nmax, x.n);
+    const int nc = kStreamCols;
+    int64_t t = 0;
+    double flops = 0, bytes = 0;
+    std::vector<StreamDesc> d;
+    for (auto& x : v) {
+      if (x.m <= 0 || x.n <= 0 || x.k1 + x.k2 <= 0) continue;
+      x.tiles_m = int(cdiv(x.m, kStreamRows));
+      x.tiles_n = int(cdiv(x.n, nc));
+      x.tile_begin = t;
+      t += int64_t(x.tiles_m) * x.tiles_n;
+      flops += 2.0 * x.m * x.n * (double(x.k1) + x.k2);
+      bytes += 8.0 * (double(x.m) * (double(x.k1) + x.k2) + (double(x.k1) + x.k2) * x.n +
+                      double(x.m) * x.n * (x.beta != 0.0 ? 2 : 1));
+      d.push_back(x);
+    }
+    if (d.empty()) return;
+    Launch L;
+    L.kind = K_STREAM; L.cat = cat; L.trans = trans; L.nmax = nmax;
+    L.off = blob.add(d); L.count = int(d.size()); L.tiles = t; L.flops = flops; L.bytes = bytes;
+    pr.l.push_back(L);
+  }
+
   // refinement convergence check over every system's residual and rhs norms
   void check(Proc& pr, int mode) {
     std::vector<CheckDesc> v;
@@ -656,10 +713,10 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
 
 // Matvec reduce: W_i = V_i^T x_i for every non-root node i (V_i: the rows of
 // node i in the column block of its depth), for the HODLR product of the
-// residual.  One product per leaf reads each V row block once,
+// residual.  One streaming product per leaf reads each V row block once,
 // P_j = V(rows of leaf j, 0 : R_D)^T x_j, and W_i is then the sum, in leaf
 // order, of the depth block of P_j over the leaves j under node i.
-void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, std::vector<Builder::G>& prod,
+void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, std::vector<StreamDesc>& prod,
                    std::vector<SumDesc>& sums) {
   const Tree& t = S.tree;
   const int64_t c = t.nrhs;
@@ -669,11 +726,11 @@ void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, s
     const TreeNode& nd = t.nodes[t.leaves[j]];
     const int64_t Rd = t.rsum[nd.depth];
     if (Rd == 0) continue;
-    Builder::G g{};
-    g.A = S.s.V + nd.row_beg; g.lda = S.s.ldv;
-    g.B = x + nd.row_beg; g.ldb = ldx;
+    StreamDesc g{};
+    g.A1 = S.s.V + nd.row_beg; g.lda1 = S.s.ldv;
+    g.B1 = x + nd.row_beg; g.ldb1 = ldx;
     g.Cout = Pb + int64_t(j) * stride; g.ldcout = S.pld;
-    g.m = int(Rd); g.n = int(c); g.k = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
+    g.m = int(Rd); g.n = int(c); g.k1 = int(nd.nrow); g.alpha = 1.0; g.beta = 0.0;
     prod.push_back(g);
   }
   for (size_t i = 1; i < t.nodes.size(); ++i) {
@@ -693,10 +750,11 @@ void matvec_reduce(hodlr_plan_s* P, SysState& S, const double* x, int64_t ldx, s
 //                  b = X[:, 0:c] (sign = -1: the residual b - A x);
 //   otherwise:     rbuf = sign * A mvx (the staged input of hodlr_matvec).
 // A x = blockdiag(K) x + sum over nodes of U_child (V_sibling^T x_sibling):
-// the diagonal-block products, then every leaf's U rows times the gathered
-// V^T x blocks of its ancestors' siblings (Z_j), all in FP64.
+// per leaf j, the rows of A x are [K_j | U_j] [x_j ; Z_j] with Z_j the
+// gathered V^T x blocks of its ancestors' siblings -- one two-segment
+// streaming product that reads K_j and U_j once -- all in FP64.
 void hodlr_product(hodlr_plan_s* P, Builder& B, Proc& pr, bool from_solution, double sign) {
-  std::vector<Builder::G> red, lk, lu;
+  std::vector<StreamDesc> red, rows;
   std::vector<SumDesc> wsum;
   std::vector<CopyDesc> gz;
   for (SysState& S : P->sys) {
@@ -710,14 +768,18 @@ void hodlr_product(hodlr_plan_s* P, Builder& B, Proc& pr, bool from_solution, do
     for (size_t j = 0; j < t.leaves.size(); ++j) {
       const int id = t.leaves[j];
       const TreeNode& nd = t.nodes[id];
-      Builder::G g{};
-      g.A = S.s.K + S.kofs[j]; g.lda = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : nd.nrow;
-      g.B = x + nd.row_beg; g.ldb = ldx;
+      const int64_t Rd = t.rsum[nd.depth];
+      StreamDesc g{};
+      g.A1 = S.s.K + S.kofs[j]; g.lda1 = S.s.klayout == HODLR_K_STACKED ? S.s.ldk : nd.nrow;
+      g.B1 = x + nd.row_beg; g.ldb1 = ldx; g.k1 = int(nd.nrow);
+      if (Rd > 0) {
+        g.A2 = S.s.X + nd.row_beg + c * S.s.ldx; g.lda2 = S.s.ldx;
+        g.B2 = Z + S.z_off[j]; g.ldb2 = Rd; g.k2 = int(Rd);
+      }
       g.Cin = from_solution ? S.s.X + nd.row_beg : nullptr; g.ldcin = S.s.ldx;
       g.Cout = R + nd.row_beg; g.ldcout = S.ldw;
-      g.m = int(nd.nrow); g.n = int(c); g.k = int(nd.nrow); g.alpha = sign; g.beta = from_solution ? 1.0 : 0.0;
-      lk.push_back(g);
-      const int64_t Rd = t.rsum[nd.depth];
+      g.m = int(nd.nrow); g.n = int(c); g.alpha = sign; g.beta = from_solution ? 1.0 : 0.0;
+      rows.push_back(g);
       if (Rd == 0) continue;
       // Z_j rows of depth l: w of the sibling of the depth-l ancestor
       int a = id;
@@ -733,19 +795,12 @@ void hodlr_product(hodlr_plan_s* P, Builder& B, Proc& pr, bool from_solution, do
         gz.push_back(cd);
         a = an.parent;
       }
-      Builder::G u{};
-      u.A = S.s.X + nd.row_beg + c * S.s.ldx; u.lda = S.s.ldx;
-      u.B = Z + S.z_off[j]; u.ldb = Rd;
-      u.Cin = R + nd.row_beg; u.ldcin = S.ldw; u.Cout = R + nd.row_beg; u.ldcout = S.ldw;
-      u.m = int(nd.nrow); u.n = int(c); u.k = int(Rd); u.alpha = sign; u.beta = 1.0;
-      lu.push_back(u);
     }
   }
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, true, 1, red);
+  B.few_cols(pr, CAT_MATVEC_VTX, true, red);
   B.sum(pr, CAT_MATVEC, wsum);
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lk);
   B.copy(pr, CAT_MATVEC, F64, F64, gz);
-  B.gemm(pr, CAT_MATVEC, F64, F64, F64, F64, false, 0, lu);
+  B.few_cols(pr, CAT_MATVEC_ROWS, false, rows);
 }
 
 void build_all(hodlr_plan_s* P, Builder& B) {
@@ -1019,7 +1074,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     // every kind a sweep holds takes a gate; a launch that ignored it would
     // run in the sweeps the check has switched off
     if (L.kind != K_GEMM && L.kind != K_SUM && L.kind != K_COPY && L.kind != K_TRSM && L.kind != K_NORM &&
-        L.kind != K_NORM_FIN && L.kind != K_CHECK)
+        L.kind != K_NORM_FIN && L.kind != K_CHECK && L.kind != K_STREAM)
       B.bad_gate = true;
   }
 }
@@ -1046,6 +1101,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s, gate);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
+    case K_STREAM: return launch_stream(L.trans, L.nmax, (const StreamDesc*)d, L.count, L.tiles, s, gate);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
@@ -1078,45 +1134,29 @@ hipEvent_t take_event(hodlr_plan_s* P) {
 }
 
 int harvest(hodlr_plan_s* P) {
-  hodlr_plan_s::Graph* graphs[3] = {&P->g_factor, &P->g_solve, &P->g_sweep};
-  bool armed = false;
-  for (auto* g : graphs) armed = armed || g->armed;
-  if (P->pending.empty() && !armed) return HODLR_OK;
+  if (P->pending.empty()) return HODLR_OK;
   if (hipStreamSynchronize(P->stream) != hipSuccess) return HODLR_ERR_HIP;
-  auto take = [&](const hodlr_plan_s::Pending& q) {
+  for (auto& q : P->pending) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) return false;
+    if (hipEventElapsedTime(&ms, q.a, q.b) != hipSuccess) return HODLR_ERR_HIP;
     P->prof_ms[q.cat] += ms;
     P->prof_n[q.cat] += 1;
     P->prof_flops[q.cat] += q.flops;
     P->prof_bytes[q.cat] += q.bytes;
-    return true;
-  };
-  for (auto& q : P->pending) {
-    if (!take(q)) return HODLR_ERR_HIP;
     P->pool.push_back(q.a);
     P->pool.push_back(q.b);
   }
   P->pending.clear();
-  for (auto* g : graphs) {
-    if (!g->armed) continue;
-    g->armed = false;
-    for (auto& q : g->ev)
-      if (!take(q)) return HODLR_ERR_HIP;
-  }
   return HODLR_OK;
 }
 
-// Drops one captured graph (and the event pairs recorded in it).
+// Drops one captured graph.
 void drop_one(hodlr_plan_s* P, hodlr_plan_s::Graph& g) {
-  (void)harvest(P);
-  if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  g.exec = nullptr;
-  for (auto& q : g.ev) {
-    P->pool.push_back(q.a);
-    P->pool.push_back(q.b);
+  if (g.exec) {
+    (void)hipStreamSynchronize(P->stream);
+    (void)hipGraphExecDestroy(g.exec);
   }
-  g.ev.clear();
+  g.exec = nullptr;
   g.sweeps = 0;
 }
 
@@ -1134,9 +1174,7 @@ template <class F> int capture(hodlr_plan_s* P, hodlr_plan_s::Graph& g, F&& body
   drop_one(P, g);
   hipGraph_t gr;
   if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HODLR_ERR_HIP;
-  P->cap_ev = &g.ev;
   int rc = body();
-  P->cap_ev = nullptr;
   if (hipStreamEndCapture(P->stream, &gr) != hipSuccess) return HODLR_ERR_HIP;
   if (rc || hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) != hipSuccess) {
     (void)hipGraphDestroy(gr);
@@ -1147,12 +1185,8 @@ template <class F> int capture(hodlr_plan_s* P, hodlr_plan_s::Graph& g, F&& body
   return HODLR_OK;
 }
 
-// Replays g; the event pairs of its previous replay are read first.
 int replay(hodlr_plan_s* P, hodlr_plan_s::Graph& g) {
-  if (g.armed && harvest(P)) return HODLR_ERR_HIP;
-  if (hipGraphLaunch(g.exec, P->stream) != hipSuccess) return HODLR_ERR_HIP;
-  g.armed = !g.ev.empty();
-  return HODLR_OK;
+  return hipGraphLaunch(g.exec, P->stream) == hipSuccess ? HODLR_OK : HODLR_ERR_HIP;
 }
 
 int run(hodlr_plan_s* P, const Proc& pr) {
@@ -1172,7 +1206,7 @@ int run(hodlr_plan_s* P, const Proc& pr) {
       if (hipEventRecord(a, P->stream) != hipSuccess) return HODLR_ERR_HIP;
       if (dispatch(P, L) != hipSuccess) return HODLR_ERR_HIP;
       if (hipEventRecord(b, P->stream) != hipSuccess) return HODLR_ERR_HIP;
-      (P->cap_ev ? *P->cap_ev : P->pending).push_back({int(L.cat), a, b, L.flops, L.bytes});
+      P->pending.push_back({int(L.cat), a, b, L.flops, L.bytes});
     } else if (dispatch(P, L) != hipSuccess) {
       return HODLR_ERR_HIP;
     }
@@ -1261,7 +1295,7 @@ int sync_stream(hodlr_plan_s* P) {
 }
 
 int run_factor(hodlr_plan_s* P) {
-  if (P->use_graph) {
+  if (P->graphs()) {
     if (!P->g_factor.exec) {
       int rc = capture(P, P->g_factor, [&] { return run(P, P->factor); });
       if (rc) return rc;
@@ -1290,7 +1324,7 @@ int run_solve(hodlr_plan_s* P, double tol, int maxiter) {
   if (hipMemcpyAsync(dctl, &P->ctl_host, sizeof(RefineCtl), hipMemcpyHostToDevice, P->stream) != hipSuccess)
     return HODLR_ERR_HIP;
   int rc = HODLR_OK;
-  if (P->use_graph) {
+  if (P->graphs()) {
     const int S = std::min(P->sweep_hint, P->ctl_host.maxiter);
     if (!P->g_solve.exec || P->g_solve.sweeps != S) {
       rc = capture(P, P->g_solve, [&] {
@@ -1314,7 +1348,7 @@ int run_solve(hodlr_plan_s* P, double tol, int maxiter) {
         sync_stream(P))
       return HODLR_ERR_HIP;
     if (!P->ctl_host.active) break;
-    if (P->use_graph) {
+    if (P->graphs()) {
       if (!P->g_sweep.exec && (rc = capture(P, P->g_sweep, [&] { return run(P, P->sweep); }))) return rc;
       rc = replay(P, P->g_sweep);
     } else {
@@ -1869,8 +1903,7 @@ int hodlr_tune_force(hodlr_plan_t P, int32_t cfg) {
 
 int hodlr_profile_enable(hodlr_plan_t P, int32_t mask) {
   if (!P) return HODLR_ERR_ARG;
-  // a captured graph holds the event pairs of the mask it was captured with
-  if (uint32_t(mask) != P->prof_mask) drop_graph(P);
+  // (captured graphs stay; they are bypassed while the mask is non-zero)
   P->prof_mask = uint32_t(mask);
   return HODLR_OK;
 }

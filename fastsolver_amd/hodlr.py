@@ -20,7 +20,7 @@ from ._lib import PlanInfo, SolveReport, System, check, device_lib, gen_lib
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
-PROFILE_NCAT = 20   # HODLR_PROFILE_NCAT
+PROFILE_NCAT = 22   # HODLR_PROFILE_NCAT
 K_PACKED = 0
 K_STACKED = 1
 

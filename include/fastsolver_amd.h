@@ -211,11 +211,11 @@ int hodlr_use_graph(hodlr_plan_t plan, int32_t enable);
  * (bit i = category i, named by hodlr_profile_category(i), i <
  * HODLR_PROFILE_NCAT), every launch of that category is bracketed by HIP
  * events recorded on the plan's stream, without synchronising; read/reset
- * harvest them.  With graph replay on, the event pairs are recorded by nodes
- * of the captured graph and read back after each replay (changing the mask
- * re-captures).  Also reports the algorithmic flops and bytes of the timed
- * launches. */
-#define HODLR_PROFILE_NCAT 20
+ * harvest them.  While any bit is set, launches run eagerly, not from a
+ * captured graph (event pairs recorded as graph nodes did not bracket their
+ * kernels reliably).  Also reports the algorithmic flops and bytes of the
+ * timed launches. */
+#define HODLR_PROFILE_NCAT 22
 int hodlr_profile_enable(hodlr_plan_t plan, int32_t mask);
 int hodlr_profile_reset(hodlr_plan_t plan);
 int hodlr_profile_read(hodlr_plan_t plan, double* times_ms, int32_t* launches, double* flops, double* bytes);
