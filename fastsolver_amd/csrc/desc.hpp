@@ -117,6 +117,19 @@ struct StreamDesc {
   int64_t tile_begin;
 };
 
+// dst = the bf16 MFMA A fragments of src^T (src: rows x cols fp64, ld lds):
+// fragment (kb, g) -- rows 32kb .. 32kb+31, columns 16g .. 16g+15 of src --
+// is 1 KB at dst + ((kb * ngroups + g) * 64 + lane) * 16 bytes, lane p + 16q
+// holding src(32kb + 8q + e, 16g + p) for e = 0..7; zero past rows / cols.
+// One workgroup per kFragOctets / 64 fragments of one column group.
+struct FragDesc {
+  const double* src;
+  int64_t lds;
+  void* dst;
+  int32_t rows, cols, ngroups, pad0;
+  int64_t tile_begin;
+};
+
 // Device-side state of the refinement loop of one plan (hodlr_solve).  The
 // host writes tol and maxiter before a solve; the check kernel keeps the rest.
 struct RefineCtl {

@@ -382,6 +382,143 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   }
 }
 
+// ========================================================= bf16 level reduce
+// The bf16 plans' level reduce W = V_i^T X_i[:, 0:n] (src/solver/gemm.cc:
+// 315-384), also on c columns for the correction solves: A = the node's V
+// block (k x m) as bf16 MFMA A fragments of V^T (FragDesc layout: a wave's
+// fragment is one contiguous 1 KB, and a k step's fragments are adjacent, so
+// the loads stay on few pages however long V's columns are), B = the node's
+// rows of X in fp32, W in fp64 (or the split-K chunk's partial).  It is HBM-bound -- X in fp32 is most of the
+// bytes -- so there is no LDS in the k loop: a workgroup owns 128 rows x 32
+// columns of W and one split-K chunk (a multiple of 32 rows), its four waves
+// take interleaved 32-row k steps, and each wave loads its MFMA operands
+// straight from global memory (16 bytes of V -- 8 consecutive rows of one
+// column -- and 32 bytes of X, rounded to bf16, per lane and fragment), the
+// next step's loads in flight while a step computes.  The four partial tiles
+// are added in LDS in wave order (deterministic).
+//   A fragment, block b, lane (p, q): V(k0 + 8q + e, r0 + 16b + p), e < 8
+//     (GemmDesc::lda holds the fragment layout's column-group count)
+//   B fragment, block c:              X(k0 + 8q + e, c0 + 16c + p)
+//   accumulator v of lane (p, q), block (b, c): W(r0 + 16b + 4q + v, c0 + 16c + p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void reduce_bf16_kernel(
+    const GemmDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  constexpr int MB = 8, NB = 2, KS = 32, RW = 16 * MB, PC = RW + 1;
+  __shared__ float s_w[4 * 16 * PC];   // per wave, one 16-column block of its partial tile
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+  const GemmDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int64_t local = bid - d.tile_begin;
+  const int per_split = d.tiles_m * d.tiles_n;
+  const int split = int(local / per_split), rem = int(local % per_split);
+  const int r0 = (rem % d.tiles_m) * RW, c0 = (rem / d.tiles_m) * 16 * NB;
+  const int kb = split * d.kchunk, ke = min(d.k, kb + d.kchunk);
+  const int m = d.m, n = d.n;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = lane & 15, q = lane >> 4;
+  // column groups past V's last read its last group, columns past n column
+  // n - 1 of X: their results are never written
+  const int ng = int(d.lda);
+  const HODLR_GLOBAL bf16x8* arow[MB];
+#pragma unroll
+  for (int b = 0; b < MB; ++b) arow[b] = (const HODLR_GLOBAL bf16x8*)glob((const bf16*)d.A) + min(r0 / 16 + b, ng - 1) * 64 + lane;
+  const int64_t kstride = int64_t(ng) * 64;   // bf16x8 slots per 32-row block
+  const HODLR_GLOBAL float* bcol[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) bcol[c] = glob((const float*)d.B) + int64_t(min(c0 + 16 * c + p, n - 1)) * d.ldb + 8 * q;
+  f32x4 acc[MB][NB];
+#pragma unroll
+  for (int b = 0; b < MB; ++b)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) acc[b][c] = f32x4{0, 0, 0, 0};
+  auto load = [&](bf16x8 (&a)[MB], float (&bb)[NB][8], int k0) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b) a[b] = arow[b][(k0 >> 5) * kstride];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bb[c][e] = bcol[c][k0 + e];
+  };
+  // the chunk's last, partial step: rows of X past ke are 0 (the V rows
+  // there are finite: the next chunk's rows, or the copy's zero padding)
+  auto load_tail = [&](bf16x8 (&a)[MB], float (&bb)[NB][8], int k0) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b) a[b] = arow[b][(k0 >> 5) * kstride];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bb[c][e] = *(k0 + 8 * q + e < ke ? bcol[c] + (k0 + e) : zero_page<float>());
+  };
+  auto compute = [&](const bf16x8 (&a)[MB], const float (&bb)[NB][8]) {
+    bf16x8 bv[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bv[c][e] = bf16(bb[c][e]);
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], bv[c], acc[b][c], 0, 0, 0);
+  };
+  // this wave's steps: k0 = kb + KS (wave + 4 j), j < cnt; the chunk's
+  // partial step, if any, is the last step of one wave
+  const int nsteps = ke > kb ? (ke - kb + KS - 1) / KS : 0;
+  const int cnt = nsteps > wave ? (nsteps - wave + 3) / 4 : 0;
+  const bool tail = cnt > 0 && (ke - kb) % KS != 0 && wave + 4 * (cnt - 1) == nsteps - 1;
+  const int nfull = cnt - (tail ? 1 : 0);
+  auto k0_of = [&](int j) { return kb + KS * (wave + 4 * j); };
+  // full steps with loads issued two steps ahead through three register
+  // sets (the straight-line body keeps two sets' loads in flight across each
+  // product); loads past the last full step repeat it (never computed)
+  bf16x8 a0[MB], a1[MB], a2[MB];
+  float b0[NB][8], b1[NB][8], b2[NB][8];
+  if (nfull > 0) {
+    auto kk = [&](int j) { return k0_of(min(j, nfull - 1)); };
+    load(a0, b0, kk(0));
+    load(a1, b1, kk(1));
+    int j = 0;
+    for (; j + 3 <= nfull; j += 3) {
+      load(a2, b2, kk(j + 2));
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(a0, b0, kk(j + 3));
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      load(a1, b1, kk(j + 4));
+      __builtin_amdgcn_sched_barrier(0);
+      compute(a2, b2);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (j < nfull) compute(a0, b0);
+    if (j + 1 < nfull) compute(a1, b1);
+  }
+  if (tail) {
+    load_tail(a0, b0, k0_of(nfull));
+    compute(a0, b0);
+  }
+  // the four partial tiles added in wave order, 16 columns at a time
+  HODLR_GLOBAL double* out = glob((double*)d.Cout + (d.nsplit > 1 ? int64_t(split) * d.split_stride : 0));
+  const double alpha = d.alpha;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s_w[(wave * 16 + p) * PC + 16 * b + 4 * q + v] = acc[b][c][v];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 16 * RW; e += kThreads) {
+      const int row = e % RW, col = e / RW;
+      float s = 0.f;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) s += s_w[(w * 16 + col) * PC + row];
+      const int gr = r0 + row, gc = c0 + 16 * c + col;
+      if (gr < m && gc < n) out[gr + int64_t(gc) * d.ldcout] = alpha * double(s);
+    }
+    __syncthreads();
+  }
+}
+
 // split-K chunks are added in chunk order: deterministic, no atomics
 __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict__ descs, int ndesc,
                                                        const int32_t* __restrict__ gate) {
@@ -405,6 +542,36 @@ __global__ __launch_bounds__(kThreads) void sum_kernel(const SumDesc* __restrict
 }
 
 // ======================================================================= copy
+// storage conversion of a copy; bf16 rounds through fp32, as the staged
+// bf16 products round their fp64 operands
+template <class TD> __device__ __forceinline__ TD store_as(double x) { return TD(x); }
+template <> __device__ __forceinline__ bf16 store_as<bf16>(double x) { return bf16(float(x)); }
+
+// V^T as bf16 MFMA A fragments (FragDesc): one thread per 16-byte lane slot,
+// 8 consecutive rows of one column read (64 bytes) and rounded through fp32.
+// A workgroup writes the kFragOctets / 64 fragments of one column group and
+// consecutive row blocks, so it reads 16 columns only (few pages of src).
+__global__ __launch_bounds__(kThreads) void frag_bf16_kernel(const FragDesc* __restrict__ descs, int ndesc) {
+  constexpr int FPW = kFragOctets / 64;   // fragments per workgroup
+  const int64_t bid = blockIdx.x;
+  const FragDesc d = descs[find_desc(descs, ndesc, bid)];
+  const int64_t t = bid - d.tile_begin;
+  const int g = int(t % d.ngroups);
+  const int kb = int(t / d.ngroups) * FPW + int(threadIdx.x) / 64;
+  if (kb >= (d.rows + 31) / 32) return;
+  const int lane = int(threadIdx.x) % 64, p = lane & 15, q = lane >> 4;
+  const int64_t o = (int64_t(kb) * d.ngroups + g) * 64 + lane;   // lane slot
+  const int col = 16 * g + p, r0 = 32 * kb + 8 * q;
+  const HODLR_GLOBAL double* src = glob(d.src) + int64_t(min(col, d.cols - 1)) * d.lds;
+  double v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = *(col < d.cols && r0 + e < d.rows ? src + (r0 + e) : zero_page<double>());
+  bf16x8 w;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) w[e] = bf16(float(v[e]));
+  ((HODLR_GLOBAL bf16x8*)glob((bf16*)d.dst))[o] = w;
+}
+
 template <class TS, class TD>
 __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restrict__ descs, int ndesc,
                                                         const int32_t* __restrict__ gate) {
@@ -446,7 +613,7 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const CopyDesc* __restri
     double x = d.alpha * double(v[u]);
     if (useb) x += d.beta * double(w[u]);
     if (ii[u] == jj[u]) x += d.diag;
-    dst[ii[u] + jj[u] * d.ldd] = TD(x);
+    dst[ii[u] + jj[u] * d.ldd] = store_as<TD>(x);
   }
 }
 
@@ -1514,6 +1681,18 @@ hipError_t launch_stream(bool trans, int ncols_max, const StreamDesc* d, int n, 
   return hipGetLastError();
 }
 
+hipError_t launch_frag_bf16(const FragDesc* d, int n, int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(frag_bf16_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_bf16(const GemmDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
+  if (n <= 0 || tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_bf16_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
+  return hipGetLastError();
+}
+
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(sum_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
@@ -1527,6 +1706,7 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
   else if (ts == F64 && td == F32) hipLaunchKernelGGL((copy_kernel<double, float>), g, b, 0, s, d, n, gate);
   else if (ts == F32 && td == F64) hipLaunchKernelGGL((copy_kernel<float, double>), g, b, 0, s, d, n, gate);
   else if (ts == F32 && td == F32) hipLaunchKernelGGL((copy_kernel<float, float>), g, b, 0, s, d, n, gate);
+  else if (ts == F64 && td == BF16) hipLaunchKernelGGL((copy_kernel<double, bf16>), g, b, 0, s, d, n, gate);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -6,7 +6,9 @@
 
 namespace hodlr {
 
-enum DT { F64 = 0, F32 = 1, BF16 = 2 };   // BF16: math type only (operands stored as F32/F64)
+// BF16: the math type of the bf16 plans' products; as a storage type only for the
+// bf16 copies of V (copy destination, reduce operand)
+enum DT { F64 = 0, F32 = 1, BF16 = 2 };
 
 constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
 constexpr int kGemmBK32 = 16;       // k step of the 64-wide tiles in fp32
@@ -35,6 +37,14 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int t
 constexpr int kPanelKMax = 128;
 hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
                              const int32_t* gate = nullptr);
+// fp64 -> bf16 MFMA A fragments of the transpose (FragDesc), kFragOctets lane slots per workgroup
+constexpr int kFragOctets = 256;
+hipError_t launch_frag_bf16(const FragDesc* d, int n, int64_t tiles, hipStream_t s);
+// bf16 plans' reduce W = A^T B (GemmDesc, trans): A in the FragDesc fragment layout of A^T
+// (lda = its column-group count), B fp32, C fp64; tiles of kReduceRows x kReduceCols, split-K
+// chunks (kchunk) a multiple of 32 rows
+constexpr int kReduceRows = 128, kReduceCols = 32;
+hipError_t launch_reduce_bf16(const GemmDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate = nullptr);
 // FP64 streaming products (StreamDesc), tiles of kStreamRows rows x kStreamCols columns;
 // `ncols_max` = the widest product of the batch (at most kStreamCols)
 constexpr int kStreamRows = 256, kStreamCols = 16;

@@ -55,7 +55,7 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
 enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS,
-            K_CHECK, K_STREAM };
+            K_CHECK, K_STREAM, K_FRAG };
 
 struct Launch {
   Kind kind;
@@ -63,11 +63,12 @@ struct Launch {
   DT t[4];
   bool trans = false;
   int role = 0;
-  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32, 3 = row panel
+  int tile = 0;          // GEMM workgroup tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 64 x 32, 3 = row panel,
+                         // 4 = bf16 reduce (kReduceRows x kReduceCols)
   int gid = -1;          // tunable launches: index into the plan's choices
   int alt[2] = {0, 1};   // tunable launches: the tile of choice 0 and of choice 1
-  size_t off_t[4] = {0, 0, 0, 0};
-  int64_t tiles_t[4] = {0, 0, 0, 0};
+  size_t off_t[5] = {0, 0, 0, 0, 0};
+  int64_t tiles_t[5] = {0, 0, 0, 0, 0};
   size_t off = 0;        // byte offset of the first descriptor in the blob
   int count = 0;
   int64_t tiles = 0;
@@ -84,7 +85,7 @@ struct Proc {
 };
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
-inline size_t esize(DT t) { return t == F64 ? 8 : 4; }
+inline size_t esize(DT t) { return t == F64 ? 8 : t == F32 ? 4 : 2; }   // storage bytes
 
 
 // Device buffer with typed offsets.
@@ -100,6 +101,8 @@ struct SysState {
   int64_t ldw = 0;                 // leading dimension of the workspaces (= N)
   size_t xws = 0, yws = 0, rbuf = 0, lu = 0, lpiv = 0, lperm = 0, linfo = 0, zbuf = 0, ldinv = 0;
   size_t vws = 0;                  // FP32 plans: V rounded to FP32 once per factor pass (N x R, ld N)
+  std::vector<size_t> vb;          // bf16 plans, per non-root node (bytes into ws): its V block (nrow x r)
+                                   //   rounded to bf16, as MFMA A fragments of V^T (FragDesc layout)
   size_t mvx = 0;                  // hodlr_matvec: the caller's x staged here (N x c FP64, ld N)
   std::vector<size_t> lu_off;      // per leaf (elements)
   std::vector<size_t> dinv_off;    // per leaf (elements): inverted 64 x 64 diagonal blocks
@@ -502,6 +505,22 @@ struct Builder {
     pr.l.push_back(L);
   }
 
+  // bf16 MFMA A fragments of transposed fp64 blocks (bf16 plans' V copies)
+  void frag(Proc& pr, Cat cat, std::vector<FragDesc> v) {
+    int64_t t = 0;
+    double bytes = 0;
+    for (auto& x : v) {
+      x.tile_begin = t;
+      const int64_t slots = cdiv(x.rows, 32) * x.ngroups * 64;
+      t += x.ngroups * cdiv(cdiv(x.rows, 32), kFragOctets / 64);   // column group x row-block run
+      bytes += double(x.rows) * x.cols * 8 + double(slots) * 16;
+    }
+    if (v.empty()) return;
+    Launch L;
+    L.kind = K_FRAG; L.cat = cat; L.off = blob.add(v); L.count = int(v.size()); L.tiles = t; L.bytes = bytes;
+    pr.l.push_back(L);
+  }
+
   // transposed bf16 copies of LU factors (bf16 plans)
   void transpose(Proc& pr, Cat cat, std::vector<TransDesc> v) {
     int64_t t = 0;
@@ -536,6 +555,68 @@ struct Builder {
     Launch F;
     F.kind = K_NORM_FIN; F.cat = CAT_NORM; F.off = off; F.count = int(v.size());
     pr.l.push_back(F);
+  }
+
+  // bf16 plans' reduce W = A^T B (A: the bf16 V copies, B: fp32 rows of X,
+  // W fp64): tiles of kReduceRows x kReduceCols, the K range cut into chunks
+  // of a multiple of 32 rows until the batch has about min_wg workgroups;
+  // chunk partials are added in order by the sum pass.  Not tunable (the
+  // tile engine does not read bf16 operands).
+  void reduce_bf16(Proc& pr, Cat cat, const std::vector<G>& gs, int64_t min_wg) {
+    double work = 0;
+    for (const G& g : gs)
+      if (g.m > 0 && g.n > 0) work += double(cdiv(g.m, kReduceRows) * cdiv(g.n, kReduceCols)) * std::max(g.k, 1);
+    const int64_t chunk = cdiv(std::max<int64_t>(2048, int64_t(work / double(min_wg))), 32) * 32;
+    std::vector<GemmDesc> d;
+    std::vector<SumDesc> sums;
+    size_t part = 0;
+    double flops = 0, bytes = 0;
+    int64_t tiles = 0;
+    for (const G& g : gs) {
+      if (g.m <= 0 || g.n <= 0 || g.k <= 0) continue;
+      GemmDesc x{};
+      x.A = g.A; x.B = g.B; x.Cout = g.Cout; x.lda = g.lda; x.ldb = g.ldb; x.ldcout = g.ldcout;
+      x.m = g.m; x.n = g.n; x.k = g.k; x.alpha = g.alpha; x.beta = 0.0;
+      int nsplit = g.k >= 2 * chunk ? int(cdiv(g.k, chunk)) : 1;
+      x.kchunk = int(cdiv(cdiv(g.k, nsplit), 32) * 32);
+      x.nsplit = nsplit = int(cdiv(g.k, x.kchunk));
+      if (nsplit > 1) {
+        x.Cout = (void*)(uintptr_t)(part * 8);   // patched to the scratch base below
+        x.ldcout = g.m;
+        x.split_stride = int64_t(g.m) * g.n;
+        SumDesc sd{};
+        sd.P = (const void*)(uintptr_t)(part * 8);
+        sd.stride = x.split_stride;
+        sd.out = g.Cout;
+        sd.ldo = g.ldcout;
+        sd.m = g.m; sd.n = g.n; sd.nsplit = nsplit; sd.ldp = g.m;
+        sums.push_back(sd);
+        part += size_t(nsplit) * g.m * g.n;
+      }
+      x.tiles_m = int(cdiv(g.m, kReduceRows));
+      x.tiles_n = int(cdiv(g.n, kReduceCols));
+      x.tile_begin = tiles;
+      tiles += int64_t(x.tiles_m) * x.tiles_n * nsplit;
+      d.push_back(x);
+      flops += 2.0 * g.m * g.n * g.k;
+      bytes += double(g.k) * g.m * 2 + double(g.k) * g.n * 4 + double(g.m) * g.n * 8;
+    }
+    if (d.empty()) return;
+    if (part) {
+      scratch_need = std::max(scratch_need, part);
+      for (auto& x : d)
+        if (x.nsplit > 1) x.Cout = (void*)(P->scratch.p + (uintptr_t)x.Cout);
+      for (auto& s : sums) s.P = (const void*)(P->scratch.p + (uintptr_t)s.P);
+    }
+    Launch L;
+    L.kind = K_GEMM; L.cat = cat;
+    L.t[0] = BF16; L.t[1] = F32; L.t[2] = F64; L.t[3] = BF16;
+    L.trans = true; L.role = 1; L.tile = 4; L.alt[0] = L.alt[1] = 4;
+    L.count = int(d.size()); L.flops = flops; L.bytes = bytes;
+    L.off = L.off_t[4] = blob.add(d);
+    L.tiles = L.tiles_t[4] = tiles;
+    pr.l.push_back(L);
+    sum(pr, cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat, sums);
   }
 
   // FP64 products with few columns (the residual and matvec sweeps), given as
@@ -633,7 +714,11 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
     S.yws = L.take(size_t(n) * c * we);
     S.rbuf = L.take(size_t(n) * c * 8);
     S.mvx = L.take(size_t(n) * c * 8);
-    if (we != 8) S.vws = L.take(size_t(n) * std::max<int64_t>(t.vcols(), 1) * we);
+    if (we != 8 && P->mt != BF16) S.vws = L.take(size_t(n) * std::max<int64_t>(t.vcols(), 1) * we);
+    S.vb.assign(t.nodes.size(), 0);
+    if (P->mt == BF16)
+      for (size_t i = 1; i < t.nodes.size(); ++i)
+        S.vb[i] = L.take(size_t(cdiv(t.nodes[i].nrow, 32) * 32) * size_t(cdiv(t.rank(t.nodes[i].depth), 16) * 16) * 2);
     S.lpiv = L.take(size_t(n) * 4);
     S.lperm = L.take(size_t(n) * 4);
     size_t lu_elems = 0, z_elems = 0, k_elems = 0;
@@ -687,8 +772,10 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
       const TreeNode& x = t.nodes[i];
       const int r = t.rank(x.depth);
       const int64_t nd = t.col_beg(x.depth);
-      S.vtd[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
-      S.vtu[i] = L.take(size_t(r) * r * 8);
+      // V^T d then V^T u, adjacent (ld r): the reduce writes both as one
+      // r x (nd + r) product (nd = c + R_{depth-1} >= c)
+      S.vtd[i] = L.take(size_t(r) * (nd + r) * 8);
+      S.vtu[i] = S.vtd[i] + size_t(r) * nd * 8;
       S.wmv[i] = L.take(size_t(r) * c * 8);
     }
     for (size_t i = 0; i < nn; ++i) {
@@ -864,7 +951,24 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       }
       leaf_base += int(t.leaves.size());
     }
-    if (wt != F64) {
+    if (P->mt == BF16) {
+      // bf16 plans: every node's V block rounded to bf16 once per factor pass
+      // (through fp32, as the products would round it), stored as the MFMA
+      // A fragments of V^T that the reduce loads directly
+      std::vector<FragDesc> vc;
+      for (SysState& S : P->sys) {
+        const Tree& t = S.tree;
+        for (size_t i = 1; i < t.nodes.size(); ++i) {
+          const TreeNode& x = t.nodes[i];
+          FragDesc f{};
+          f.src = S.s.V + x.row_beg + t.rsum[x.depth - 1] * S.s.ldv; f.lds = S.s.ldv;
+          f.dst = P->ws.p + S.vb[i];
+          f.rows = int(x.nrow); f.cols = t.rank(x.depth); f.ngroups = int(cdiv(f.cols, 16));
+          vc.push_back(f);
+        }
+      }
+      B.frag(P->leaf, CAT_LEAF_COPY, vc);
+    } else if (wt != F64) {
       // V is read by every column tile of the reduce: round it to FP32 once
       std::vector<CopyDesc> vc;
       for (SysState& S : P->sys) {
@@ -906,24 +1010,22 @@ void build_all(hodlr_plan_s* P, Builder& B) {
           double* vtu[2] = {at<double>(P, S.vtu[ch[0]]), at<double>(P, S.vtu[ch[1]])};
           double* eta0 = at<double>(P, S.eta[pi]);
           double* Sm = at<double>(P, S.sbuf[pi]);
+          // W = V^T [d | u] (mode 0; V^T d = vtd and V^T u = vtu adjacent,
+          // X's d and u columns adjacent: one product) or V^T y (mode 1)
           for (int k = 0; k < 2; ++k) {
             const TreeNode& C = t.nodes[ch[k]];
             Builder::G g{};
-            if (wt == F64) {
+            if (P->mt == BF16) {
+              g.A = P->ws.p + S.vb[ch[k]]; g.lda = cdiv(r, 16);   // fragment column groups
+            } else if (wt == F64) {
               g.A = S.s.V + C.row_beg + vcol * S.s.ldv; g.lda = S.s.ldv;
             } else {
               g.A = P->ws.p + S.vws + (size_t(C.row_beg) + size_t(vcol) * S.ldw) * we; g.lda = S.ldw;
             }
             g.B = X + size_t(C.row_beg) * we; g.ldb = S.ldw;
             g.Cout = vtd[k]; g.ldcout = r; g.Cin = nullptr; g.ldcin = r;
-            g.m = r; g.n = nd; g.k = int(C.nrow); g.alpha = 1.0; g.beta = 0.0;
+            g.m = r; g.n = mode == 0 ? nd + r : nd; g.k = int(C.nrow); g.alpha = 1.0; g.beta = 0.0;
             red.push_back(g);
-            if (mode == 0) {
-              Builder::G u = g;
-              u.B = X + (size_t(C.row_beg) + size_t(ucol) * S.ldw) * we;
-              u.Cout = vtu[k]; u.n = r;
-              red.push_back(u);
-            }
           }
           // node: VTd1 -= VTu1 VTd0
           Builder::G a{};
@@ -993,7 +1095,8 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       const Cat cg = mode == 0 ? CAT_NODE_GEMM : CAT_SOLVE_NODE;
       const Cat ct = mode == 0 ? CAT_NODE_TRSM : CAT_SOLVE_NODE;
       const Cat cb = mode == 0 ? CAT_BCAST : CAT_SOLVE_BCAST;
-      B.gemm(pr, cr, wt, wt, F64, P->mt, true, 1, red, 2048);
+      if (P->mt == BF16) B.reduce_bf16(pr, cr, red, 2048);
+      else B.gemm(pr, cr, wt, wt, F64, P->mt, true, 1, red, 2048);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng1);
       if (mode == 0) {
         B.copy(pn, CAT_NODE_LU, F64, F64, sform);
@@ -1094,6 +1197,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
       }
       const GemmDesc* gd = (const GemmDesc*)(P->descs.p + L.off_t[tile]);
       if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], gd, L.count, L.tiles_t[3], s, gate);
+      if (tile == 4) return launch_reduce_bf16(gd, L.count, L.tiles_t[4], s, gate);
       return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s,
                          gate);
     }
@@ -1102,6 +1206,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
     case K_STREAM: return launch_stream(L.trans, L.nmax, (const StreamDesc*)d, L.count, L.tiles, s, gate);
+    case K_FRAG: return launch_frag_bf16((const FragDesc*)d, L.count, L.tiles, s);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
