@@ -22,8 +22,9 @@
 namespace hodlr {
 
 // ======================================================================= GEMM
-// ROLE only separates the symbol names of the callers in profiles:
-// 0 node/matvec products, 1 reduce V^T X, 2 broadcast d -= u eta, 3 LU trailing update.
+// ROLE only separates the symbol names of the callers in profiles: 0 node and
+// correction products, 1 reduce V^T X, 2 broadcast d -= u eta, 3 LU trailing
+// update, 4 the FP64 HODLR matvec of the residuals.
 template <int ROLE, class TA, class TB, class TC, class MT, bool TRANS, int BM, int BN>
 __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc,
                                                            const int32_t* __restrict__ gate) {
@@ -1632,7 +1633,15 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
     else HODLR_GEMM(R, A, B, C, M, T, 64, 64);                         \
   } while (0)
   if (ta == F64 && tb == F64 && tc == F64 && mt == F64) {
-    if (trans) HODLR_GEMM3(1, double, double, double, double, true);
+    if (role == 4) {
+      if (tile == 2) {
+        if (trans) HODLR_GEMM(4, double, double, double, double, true, 64, kGemmNarrowBN);
+        else HODLR_GEMM(4, double, double, double, double, false, 64, kGemmNarrowBN);
+      } else {
+        if (trans) HODLR_GEMM(4, double, double, double, double, true, 64, 64);
+        else HODLR_GEMM(4, double, double, double, double, false, 64, 64);
+      }
+    } else if (trans) HODLR_GEMM3(1, double, double, double, double, true);
     else if (role == 2) HODLR_GEMM3(2, double, double, double, double, false);
     else if (role == 3) HODLR_GEMM3(3, double, double, double, double, false);
     else if (tile == 2) HODLR_GEMM(0, double, double, double, double, false, 64, kGemmNarrowBN);

@@ -302,7 +302,7 @@ struct Builder {
         L.tiles_t[3] = tiles;
         continue;
       }
-      if (narrow ? cfg != 2 : (cfg == 2 || (cfg == 1 && role == 0))) continue;
+      if (narrow ? cfg != 2 : (cfg == 2 || (cfg == 1 && (role == 0 || role == 4)))) continue;
       const int TM = cfg == 1 ? 128 : kGemmBM, TN = cfg == 1 ? 128 : (cfg == 2 ? kGemmNarrowBN : kGemmBN);
       int64_t tiles = 0;
       for (auto& x : d) {
@@ -319,7 +319,7 @@ struct Builder {
       L.alt[0] = 3;
       L.alt[1] = narrow ? 2 : 0;
       if (tunable) L.gid = int(ngid++);
-    } else if (role >= 1 && tunable && !narrow) {
+    } else if (role >= 1 && role <= 3 && tunable && !narrow) {
       L.gid = int(ngid++);
     }
     L.off = L.off_t[L.tile];
@@ -645,8 +645,8 @@ struct Builder {
         g2.push_back(b);
       }
     }
-    gemm(pr, cat, F64, F64, F64, F64, trans, trans ? 1 : 0, g1);
-    gemm(pr, cat, F64, F64, F64, F64, false, 0, g2);
+    gemm(pr, cat, F64, F64, F64, F64, trans, 4, g1);
+    gemm(pr, cat, F64, F64, F64, F64, false, 4, g2);
   }
 
   // FP64 streaming products, tiles of kStreamRows rows x kStreamCols columns
