@@ -161,34 +161,28 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
       }
     }
   }
-  // the chunk's B fragments: column 16w + li, k = 32 ks + 8 lq .. + 7
-  f32x4 bv[NKS][2];
-  auto load_b = [&](int c0) {
-    const int col = c0 + 16 * wave + li;
-    const HODLR_GLOBAL f32x4* bc = (const HODLR_GLOBAL f32x4*)(B + int64_t(col < n ? col : 0) * ldb);
-    const HODLR_GLOBAL f32x4* z = (const HODLR_GLOBAL f32x4*)zero_page<double>();
+  // the chunk's B fragments: column 16w + li, k = 32 ks + 8 lq .. + 7.
+  // Columns past n read column n - 1 and rows past m row m - 1 (their results
+  // are never stored), so the chunk loads need no branch.
+  const HODLR_GLOBAL f32x4* zb = (const HODLR_GLOBAL f32x4*)zero_page<double>();
+  auto load_b = [&](f32x4 (&bv)[NKS][2], int c0) {
+    const int col = min(c0 + 16 * wave + li, n - 1);
+    const HODLR_GLOBAL f32x4* bc = (const HODLR_GLOBAL f32x4*)(B + int64_t(col) * ldb);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       const int kk = 32 * ks + 8 * lq;
-      const bool ok = col < n && ks < nks && kk + 8 <= ldb;
-      bv[ks][0] = *(ok ? bc + kk / 4 : z);
-      bv[ks][1] = *(ok ? bc + kk / 4 + 1 : z + 1);
+      const bool ok = ks < nks && kk + 8 <= ldb;
+      bv[ks][0] = *(ok ? bc + kk / 4 : zb);
+      bv[ks][1] = *(ok ? bc + kk / 4 + 1 : zb + 1);
     }
   };
-  float cv[CPT];
-  auto load_c = [&](int c0) {
+  const HODLR_GLOBAL float* crow = Cin + min(r0 + fr, m - 1);
+  auto load_c = [&](float (&cv)[CPT], int c0) {
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      const int col = c0 + fc + 4 * q;
-      cv[q] = *(rok && col < n ? Cin + (r0 + fr) + int64_t(col) * ldi : zero_page<float>());
-    }
+    for (int q = 0; q < CPT; ++q) cv[q] = crow[int64_t(min(c0 + fc + 4 * q, n - 1)) * ldi];
   };
-  load_b(0);
-  load_c(0);
-  lds_barrier();
-
-  for (int c0 = 0; c0 < n; c0 += BN) {
-    // product: acc[mb] = A(rows of block mb) x B(this wave's 16 columns)
+  // one chunk: product, accumulators through LDS, C written coalesced
+  auto chunk = [&](int c0, const f32x4 (&bv)[NKS][2], const float (&cv)[CPT]) {
     f32x4 acc[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x4{0, 0, 0, 0};
@@ -206,26 +200,35 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
           acc[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a[(mb * NKS + ks) * 64 + lane], b, acc[mb], 0, 0, 0);
       }
     }
-    // accumulators -> LDS [col][row]; the next chunk's loads go out once this
-    // chunk's C values are taken
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) s_c[(16 * wave + li) * PC + 16 * mb + 4 * lq + v] = acc[mb][v];
-    float cnow[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) cnow[q] = cv[q];
-    if (c0 + BN < n) {
-      load_b(c0 + BN);
-      load_c(c0 + BN);
-    }
     lds_barrier();
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       const int col = c0 + fc + 4 * q;
-      if (rok && col < n) Cout[(r0 + fr) + int64_t(col) * ldo] = alpha * s_c[(fc + 4 * q) * PC + fr] + beta * cnow[q];
+      if (rok && col < n) Cout[(r0 + fr) + int64_t(col) * ldo] = alpha * s_c[(fc + 4 * q) * PC + fr] + beta * cv[q];
     }
     lds_barrier();
+  };
+  // two register sets, no copies between them: the next chunk's loads are
+  // issued before a chunk's product and stay in flight through it
+  f32x4 bvA[NKS][2], bvB[NKS][2];
+  float cvA[CPT], cvB[CPT];
+  load_b(bvA, 0);
+  load_c(cvA, 0);
+  lds_barrier();
+  for (int c0 = 0; c0 < n; c0 += 2 * BN) {
+    load_b(bvB, c0 + BN);
+    load_c(cvB, c0 + BN);
+    __builtin_amdgcn_sched_barrier(0);
+    chunk(c0, bvA, cvA);
+    if (c0 + BN >= n) break;
+    load_b(bvA, c0 + 2 * BN);
+    load_c(cvA, c0 + 2 * BN);
+    __builtin_amdgcn_sched_barrier(0);
+    chunk(c0 + BN, bvB, cvB);
   }
 }
 
