@@ -114,6 +114,40 @@ def test_matvec_and_solve_only():
     assert rel(dzo.download(), z) <= TOL
 
 
+@pytest.mark.parametrize("nrhs", [1, 16, 17, 32])
+@pytest.mark.parametrize("n", [4096, 4022])
+def test_matvec_column_widths(n, nrhs):
+    """HODLR matvec against the oracle for 1-32 right-hand sides: the FP64
+    streaming products' 16- and 32-column tiles, with 32-byte vector loads
+    (n = 4096: leaves of 512 rows, aligned operands, leading dimensions
+    multiples of 4) and with scalar loads (n = 4022: leaves of 502-503 rows)."""
+    spec = gen.SystemSpec(n, 512, nrhs, [40, 24, 16], seed=1245667 + n + nrhs)
+    X, V, K = host_system(spec)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_FP64)
+    z = np.asfortranarray(np.random.default_rng(n + nrhs).standard_normal((n, nrhs)))
+    yo = oracle.hodlr_matvec(spec.as_dict(), X, V, K, z)
+    dz = hodlr.DeviceArray.from_host(z)
+    dy = hodlr.DeviceArray(n, nrhs)
+    plan.matvec([dz], [dy])
+    assert rel(dy.download(), yo) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [4096, 4022])
+def test_refined_solve_32_columns(n):
+    """BF16 factor + FP64 refinement on 32 right-hand sides: the residuals run
+    through the 32-column streaming products (vector loads at n = 4096,
+    scalar at n = 4022); solution against the oracle."""
+    spec = gen.SystemSpec(n, 512, 32, [40, 24, 16], seed=1245667 + 3 * n)
+    X, V, K = host_system(spec)
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    plan = hodlr.Plan([ds], hodlr.PREC_BF16)
+    rep = plan.solve(tol=TOL, maxiter=10)
+    assert rep.converged and rep.relres <= TOL
+    xo = oracle_solve(spec, X, V, K)[:, :32]
+    assert rel(ds.solution(), xo) <= TOL
+
+
 def test_graph_replay_matches_eager():
     spec = gen.config(1, scale=8)[0]
     X, V, K = host_system(spec)
