@@ -121,7 +121,7 @@ struct StreamDesc {
 // fragment (kb, g) -- rows 32kb .. 32kb+31, columns 16g .. 16g+15 of src --
 // is 1 KB at dst + ((kb * ngroups + g) * 64 + lane) * 16 bytes, lane p + 16q
 // holding src(32kb + 8q + e, 16g + p) for e = 0..7; zero past rows / cols.
-// One workgroup per kFragOctets / 64 fragments of one column group.
+// One workgroup per 4 kFragPerWave fragments of one column group (consecutive kb).
 struct FragDesc {
   const double* src;
   int64_t lds;

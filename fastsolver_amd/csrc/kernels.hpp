@@ -37,9 +37,11 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int t
 constexpr int kPanelKMax = 128;
 hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
                              const int32_t* gate = nullptr);
-// fp64 -> bf16 MFMA A fragments of the transpose (FragDesc), kFragOctets lane slots per workgroup
-constexpr int kFragOctets = 256;
-hipError_t launch_frag_bf16(const FragDesc* d, int n, int64_t tiles, hipStream_t s);
+// fp64 -> bf16 MFMA A fragments of the transpose (FragDesc), 4 kFragPerWave fragments per
+// workgroup (one column group, consecutive row blocks); `vec`: 32-byte loads, allowed when every
+// src is 32-byte aligned, every lds a multiple of 4 and every rows a multiple of 8
+constexpr int kFragPerWave = 4;
+hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, hipStream_t s);
 // bf16 plans' reduce W = A^T B (GemmDesc, trans): A in the FragDesc fragment layout of A^T
 // (lda = its column-group count), B fp32, C fp64; tiles of kReduceRows x kReduceCols, split-K
 // chunks (kchunk) a multiple of 32 rows

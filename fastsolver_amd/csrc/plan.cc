@@ -509,15 +509,18 @@ struct Builder {
   void frag(Proc& pr, Cat cat, std::vector<FragDesc> v) {
     int64_t t = 0;
     double bytes = 0;
+    bool vec = true;   // 32-byte loads: aligned sources, whole 8-row slots
     for (auto& x : v) {
       x.tile_begin = t;
       const int64_t slots = cdiv(x.rows, 32) * x.ngroups * 64;
-      t += x.ngroups * cdiv(cdiv(x.rows, 32), kFragOctets / 64);   // column group x row-block run
+      t += x.ngroups * cdiv(cdiv(x.rows, 32), 4 * kFragPerWave);   // column group x row-block run
       bytes += double(x.rows) * x.cols * 8 + double(slots) * 16;
+      vec = vec && (uintptr_t)x.src % 32 == 0 && x.lds % 4 == 0 && x.rows % 8 == 0;
     }
     if (v.empty()) return;
     Launch L;
     L.kind = K_FRAG; L.cat = cat; L.off = blob.add(v); L.count = int(v.size()); L.tiles = t; L.bytes = bytes;
+    L.mode = vec;
     pr.l.push_back(L);
   }
 
@@ -1217,7 +1220,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
     case K_STREAM: return launch_stream(L.trans, L.nmax, L.mode != 0, (const StreamDesc*)d, L.count, L.tiles, s, gate);
-    case K_FRAG: return launch_frag_bf16((const FragDesc*)d, L.count, L.tiles, s);
+    case K_FRAG: return launch_frag_bf16(L.mode != 0, (const FragDesc*)d, L.count, L.tiles, s);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
