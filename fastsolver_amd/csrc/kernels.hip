@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <limits>
 #include <type_traits>
 
 #include "common.hpp"
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
   __shared__ T s_x[PW * PX];          // U12 rows of the current sub-panel
   __shared__ int s_info;
   if (tid == 0) s_info = 0;
-  __syncthreads();
+  lds_barrier();
 
   for (int sp = 0; sp < NSP; ++sp) {
     const int ks = k0 + PW * sp;
@@ -807,7 +808,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
         }
         wave_argmax(bv, bi);
         if (lane == 0) { s_rv[buf][wave] = bv; s_ri[buf][wave] = bi; }
-        __syncthreads();
+        lds_barrier();
         T pv = s_rv[buf][0];
         int p = s_ri[buf][0];
 #pragma unroll
@@ -829,7 +830,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
           d.ipiv[ks + j] = ks + p;
           if (!(pv > T(0)) && s_info == 0) s_info = ks + j + 1;
         }
-        __syncthreads();
+        lds_barrier();
         if (p != j) {
           if (tid == j) {
 #pragma unroll
@@ -869,7 +870,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
 #pragma unroll
       for (int jj = 0; jj < PW; ++jj) s_l[tid * (PW + 1) + jj] = a[0][jj];
     }
-    __syncthreads();
+    lds_barrier();
     // this sub-panel's interchanges on the other panel columns, in order
     const int nleft = ks - k0, nright = k0 + nb - (ks + PW);
     for (int t = tid; t < nleft + max(nright, 0); t += NT) {
@@ -893,7 +894,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
           if (e < PW * nright) s_x[(e % PW) * PX + e / PW] = v[u];
         }
       }
-      __syncthreads();
+      lds_barrier();
       // U12 = L11^-1 A12 (unit lower), one thread per column
       if (tid < nright) {
         for (int i = 1; i < PW; ++i) {
@@ -902,7 +903,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
           s_x[i * PX + tid] = x;
         }
       }
-      __syncthreads();
+      lds_barrier();
       for (int e = tid; e < PW * nright; e += NT) {
         const int r = e % PW, c = e / PW;
         A[(ks + r) + int64_t(ks + PW + c) * lda] = s_x[r * PX + c];
@@ -962,6 +963,161 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
     }
   }
   if (tid == 0 && s_info != 0 && *d.info == 0) *d.info = s_info;
+}
+
+// Whole LU with partial pivoting of blocks of at most kSmallLU = 128 rows
+// (the node systems S, small leaves), one 1024-thread workgroup per block,
+// the block in registers: thread (ty, tx) = (t / 32, t % 32) holds
+// A(ty + 32u, tx + 32w), u, w < 4 (zero past m), so a column step costs each
+// thread at most 16 FMAs.  Column step j (its column block v = j / 32 is a
+// compile-time index, so blocks above / left of the step are skipped
+// statically):
+//   1. the owners of column j post their best candidate (first largest |a|
+//      at or below row j: idamax, NaN never wins); lanes 0..31 of every wave
+//      read the 32 candidates and combine them with DPP row shifts and a row
+//      broadcast;
+//   2. the owners of rows j and p post them; the exchange is done in
+//      registers; the owners of column j scale it by the pivot's reciprocal
+//      (dgetf2's dscal; unscaled below a zero pivot, whose column info
+//      records) and post it;
+//   3. every thread applies the rank-1 update to its entries right of and
+//      below the pivot (the posted L column is 0 at rows <= j, the U row 0 at
+//      columns <= j, so the update needs no predicate).
+// Three LDS-only barriers per column, no global memory between the load and
+// the store of the block.  Writes LU in place, ipiv and info; the inverted
+// diagonal blocks and perm come from lu_dinv_kernel and lu_perm_kernel.
+template <class T>
+__global__ __launch_bounds__(1024) void lu_small_kernel(const LuDesc* __restrict__ descs) {
+  constexpr int G = 32, NU = kSmallLU / G;   // thread grid side, blocks per thread side
+  const LuDesc d = descs[blockIdx.x];
+  const int m = d.m;
+  if (m <= 0) return;
+  HODLR_GLOBAL T* A = glob((T*)d.A);
+  const int64_t lda = d.lda;
+  const int tid = threadIdx.x, tx = tid % G, ty = tid / G, lane = tid & 63;
+  __shared__ T s_rj[kSmallLU], s_rp[kSmallLU], s_l[kSmallLU];
+  __shared__ T s_cv[G];
+  __shared__ int s_ci[G];
+  __shared__ int s_ipiv[kSmallLU];
+  T a[NU][NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int w = 0; w < NU; ++w) {
+      const int r = ty + G * u, c = tx + G * w;
+      a[u][w] = *(r < m && c < m ? A + r + int64_t(c) * lda : zero_page<T>());
+    }
+  int info = 0;
+  // column block v as a template constant (a loop the compiler might not
+  // unroll would index a[][] at run time, through scratch memory)
+  auto block = [&](auto vc) __attribute__((always_inline)) {
+    constexpr int v = decltype(vc)::value;
+    if (G * v >= m) return;
+    for (int tj = 0; tj < G && G * v + tj < m; ++tj) {
+      const int j = G * v + tj;
+      if (tx == tj) {
+        T bv = T(-1);
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int u = v; u < NU; ++u) {
+          const int r = ty + G * u;
+          const T x = fabs(a[u][v]);
+          if (r >= j && r < m && x > bv) { bv = x; bi = r; }
+        }
+        s_cv[ty] = bv;
+        s_ci[ty] = bi;
+      }
+      lds_barrier();
+      T pv = s_cv[lane & 31];
+      int p = s_ci[lane & 31];
+      argmax_step<T, 0x111, 0xf>(pv, p);   // row_shr:1
+      argmax_step<T, 0x112, 0xf>(pv, p);   // row_shr:2
+      argmax_step<T, 0x114, 0xf>(pv, p);   // row_shr:4
+      argmax_step<T, 0x118, 0xf>(pv, p);   // row_shr:8: lane 15 of each 16-lane row holds its best
+      argmax_step<T, 0x142, 0xa>(pv, p);   // row_bcast:15 into rows 1 and 3: lane 31 holds the best of 32
+      p = __builtin_amdgcn_readlane(p, 31);
+      if constexpr (sizeof(T) == 8) {
+        const long long bits = __double_as_longlong(pv);
+        const int lo = __builtin_amdgcn_readlane(int(bits), 31), hi = __builtin_amdgcn_readlane(int(bits >> 32), 31);
+        pv = __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+      } else {
+        pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 31));
+      }
+      if (p >= m) p = j;   // no candidate (all NaN): keep row j
+      if (!(pv > T(0)) && info == 0) info = j + 1;
+      if (tid == 0) s_ipiv[j] = p;
+      const int pu = p / G, pt = p % G;
+      if (ty == tj && p != j) {
+#pragma unroll
+        for (int w = 0; w < NU; ++w) s_rj[tx + G * w] = a[v][w];
+      }
+      if (ty == pt) {
+#pragma unroll
+        for (int u = v; u < NU; ++u) {
+          // (an opaque test per block: the compiler would otherwise fold the
+          // chain into a[pu][w], a run-time index into scratch memory)
+          int pick = u == pu;
+          asm volatile("" : "+v"(pick));
+          if (pick) {
+#pragma unroll
+            for (int w = 0; w < NU; ++w) s_rp[tx + G * w] = a[u][w];
+          }
+        }
+      }
+      lds_barrier();
+      if (p != j) {
+        if (ty == tj) {
+#pragma unroll
+          for (int w = 0; w < NU; ++w) a[v][w] = s_rp[tx + G * w];
+        }
+        if (ty == pt) {
+#pragma unroll
+          for (int u = v; u < NU; ++u) {
+            int pick = u == pu;
+            asm volatile("" : "+v"(pick));
+            if (pick) {
+#pragma unroll
+              for (int w = 0; w < NU; ++w) a[u][w] = s_rj[tx + G * w];
+            }
+          }
+        }
+      }
+      if (tx == tj) {
+        const T piv = s_rp[j];
+        const T rcp = piv != T(0) ? T(1) / piv : T(1);
+#pragma unroll
+        for (int u = v; u < NU; ++u) {
+          const int r = ty + G * u;
+          T l = T(0);
+          if (r > j && r < m) {
+            l = a[u][v] * rcp;
+            a[u][v] = l;
+          }
+          s_l[r] = l;
+        }
+      }
+      lds_barrier();
+#pragma unroll
+      for (int u = v; u < NU; ++u) {
+        const T lu = s_l[ty + G * u];
+#pragma unroll
+        for (int w = v; w < NU; ++w) a[u][w] -= lu * (tx + G * w > j ? s_rp[tx + G * w] : T(0));
+      }
+    }
+  };
+  block(std::integral_constant<int, 0>());
+  block(std::integral_constant<int, 1>());
+  block(std::integral_constant<int, 2>());
+  block(std::integral_constant<int, 3>());
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int w = 0; w < NU; ++w) {
+      const int r = ty + G * u, c = tx + G * w;
+      if (r < m && c < m) A[r + int64_t(c) * lda] = a[u][w];
+    }
+  for (int j = tid; j < m; j += 1024) d.ipiv[j] = s_ipiv[j];   // (the last barrier ordered s_ipiv)
+  if (tid == 0) *d.info = info;
 }
 
 // perm from the sequential interchanges ipiv (row i of P*A is row perm[i] of A)
@@ -1818,6 +1974,13 @@ hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStr
   dim3 g(nblk, n), b(128);
   if (t == F64) hipLaunchKernelGGL((lu_dinv_kernel<double>), g, b, 0, s, d, b0);
   else hipLaunchKernelGGL((lu_dinv_kernel<float>), g, b, 0, s, d, b0);
+  return hipGetLastError();
+}
+
+hipError_t launch_lu_small(DT t, const LuDesc* d, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (t == F64) hipLaunchKernelGGL(lu_small_kernel<double>, dim3(n), dim3(1024), 0, s, d);
+  else hipLaunchKernelGGL(lu_small_kernel<float>, dim3(n), dim3(1024), 0, s, d);
   return hipGetLastError();
 }
 

@@ -61,6 +61,9 @@ hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hi
                        const int32_t* gate = nullptr);
 hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, hipStream_t s);
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
+// whole LU (in place, ipiv, info; not dinv / perm) of blocks of at most kSmallLU rows, in registers
+constexpr int kSmallLU = 128;
+hipError_t launch_lu_small(DT t, const LuDesc* d, int n, hipStream_t s);
 // inverted 64 x 64 diagonal blocks b0 .. b0+nblk-1 of every factor
 hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStream_t s);
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0

@@ -54,7 +54,7 @@ const char* kCatNames[CAT_N] = {"leaf_copy", "lu_panel", "lu_rowblock", "lu_upda
                                 "solve_bcast", "matvec_vtx", "matvec_rows"};
 static_assert(CAT_N == HODLR_PROFILE_NCAT, "category count");
 
-enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS,
+enum Kind { K_GEMM, K_SUM, K_COPY, K_LU_INIT, K_LU_PANEL, K_LU_PERM, K_LU_DINV, K_LU_SMALL, K_TRSM, K_NORM, K_NORM_FIN, K_TRANS,
             K_CHECK, K_STREAM, K_FRAG };
 
 struct Launch {
@@ -73,7 +73,8 @@ struct Launch {
   int count = 0;
   int64_t tiles = 0;
   int k0 = 0, rows_max = 0, mmax = 0;
-  int mode = 0;          // K_CHECK: 0 after the first residual, 1 after a sweep
+  int mode = 0;          // K_CHECK: 0 after the first residual, 1 after a sweep; K_LU_DINV: blocks
+                         // from k0 / kLuNB to invert (0: one)
   int nmax = 0;          // K_STREAM: the widest product of the batch (columns)
   bool gated = false;    // a refinement-sweep launch: runs only while the plan's RefineCtl::active is set
   double flops = 0, bytes = 0;   // algorithmic work of the launch (profiling)
@@ -370,6 +371,22 @@ struct Builder {
     int mmax = 0;
     for (auto& x : v) mmax = std::max(mmax, x.m);
     size_t off = blob.add(v);
+    if (mmax <= kSmallLU) {
+      // every block in registers: one launch for the whole factorization
+      // (info included), then the inverted diagonal blocks and perm
+      Launch S;
+      S.kind = K_LU_SMALL; S.cat = panel_cat; S.t[0] = t; S.off = off; S.count = int(v.size());
+      for (auto& x : v) S.flops += 2.0 / 3.0 * double(x.m) * x.m * x.m;
+      pr.l.push_back(S);
+      Launch Dv;
+      Dv.kind = K_LU_DINV; Dv.cat = misc_cat; Dv.t[0] = t; Dv.off = off; Dv.count = int(v.size()); Dv.k0 = 0;
+      Dv.mode = int(cdiv(mmax, kLuNB));
+      pr.l.push_back(Dv);
+      Launch Pm;
+      Pm.kind = K_LU_PERM; Pm.cat = misc_cat; Pm.off = off; Pm.count = int(v.size());
+      pr.l.push_back(Pm);
+      return;
+    }
     Launch I;
     I.kind = K_LU_INIT; I.cat = misc_cat; I.off = off; I.count = int(v.size());
     pr.l.push_back(I);
@@ -1215,7 +1232,8 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
       return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s,
                          gate);
     }
-    case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, 1, s);
+    case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, std::max(1, L.mode), s);
+    case K_LU_SMALL: return launch_lu_small(L.t[0], (const LuDesc*)d, L.count, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s, gate);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);

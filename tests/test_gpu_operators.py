@@ -113,7 +113,7 @@ def test_bf16_gemm_integer_known_answer_and_rounding():
         assert np.abs(dC.download() - A.astype(np.float64) @ B.astype(np.float64)).max() > 1e-4
 
 
-@pytest.mark.parametrize("n", [64, 130, 200, 513, 777, 1024])
+@pytest.mark.parametrize("n", [64, 100, 128, 130, 200, 513, 777, 1024])
 def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     """The in-place LU factors equal LAPACK getrf's: the pivot search picks the
     same rows (first largest |a|, dgetf2's idamax), so L and U agree to rounding
