@@ -371,9 +371,12 @@ struct Builder {
     int mmax = 0;
     for (auto& x : v) mmax = std::max(mmax, x.m);
     size_t off = blob.add(v);
-    if (mmax <= kSmallLU) {
+    if (mmax <= kSmallLU && mmax > kLuNB) {
       // every block in registers: one launch for the whole factorization
-      // (info included), then the inverted diagonal blocks and perm
+      // (info included), then the inverted diagonal blocks and perm.  Blocks
+      // of at most 64 rows keep the one-wave panel path: a 1024-thread group
+      // per block would leave large batches of them (many small systems)
+      // with one block in flight per CU
       Launch S;
       S.kind = K_LU_SMALL; S.cat = panel_cat; S.t[0] = t; S.off = off; S.count = int(v.size());
       for (auto& x : v) S.flops += 2.0 / 3.0 * double(x.m) * x.m * x.m;
