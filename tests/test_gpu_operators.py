@@ -129,6 +129,23 @@ def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     assert np.abs(got - lu).max() <= 1e-10 * np.abs(lu).max()
 
 
+@pytest.mark.parametrize("n,zero_col", [(100, 70), (128, 0), (72, 71)])
+def test_dgesv_batched_zero_pivot_matches_lapack_getrf(n, zero_col):
+    """Blocks of 65-128 rows take the register-resident LU: a zero column
+    gives an exactly zero pivot there; info and the factors (the column left
+    unscaled, the elimination continued) equal LAPACK getrf's."""
+    import scipy.linalg.lapack as lapack
+    rng = np.random.default_rng(300 + n)
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    A[:, zero_col] = 0.0
+    lu, piv, info_ref = lapack.dgetrf(A)
+    assert info_ref == zero_col + 1
+    dA, dB = hodlr.DeviceArray.from_host(A), hodlr.DeviceArray.from_host(np.ones((n, 1)))
+    info = hodlr.dgesv_batched([dA], [dB])
+    assert info[0] == info_ref
+    assert np.abs(dA.download() - lu).max() <= 1e-10 * np.abs(lu).max()
+
+
 def test_dgesv_batched_uneven_sizes_in_one_call():
     """One batched call over blocks of different orders (the recursive LU runs
     each step over the largest block; smaller ones drop out of the late steps):
