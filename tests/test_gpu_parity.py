@@ -114,13 +114,14 @@ def test_matvec_and_solve_only():
     assert rel(dzo.download(), z) <= TOL
 
 
-@pytest.mark.parametrize("nrhs", [1, 16, 17, 32])
+@pytest.mark.parametrize("nrhs", [1, 16, 17, 32, 33])
 @pytest.mark.parametrize("n", [4096, 4022])
 def test_matvec_column_widths(n, nrhs):
-    """HODLR matvec against the oracle for 1-32 right-hand sides: the FP64
+    """HODLR matvec against the oracle for 1-33 right-hand sides: the FP64
     streaming products' 16- and 32-column tiles, with 32-byte vector loads
     (n = 4096: leaves of 512 rows, aligned operands, leading dimensions
-    multiples of 4) and with scalar loads (n = 4022: leaves of 502-503 rows)."""
+    multiples of 4) and with scalar loads (n = 4022: leaves of 502-503 rows);
+    past 32 columns the tile engine's products."""
     spec = gen.SystemSpec(n, 512, nrhs, [40, 24, 16], seed=1245667 + n + nrhs)
     X, V, K = host_system(spec)
     ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
