@@ -120,6 +120,7 @@ __global__ __launch_bounds__(kThreads, 3) void gemm_kernel(const GemmDesc* __res
 // and written coalesced, the next chunk's loads in flight while a chunk
 // computes.  HBM traffic: C read + write plus A once.
 // Wave w computes columns 16w .. 16w+15 of a chunk for all 64 rows.
+template <bool BFRAG>
 __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __restrict__ descs, int ndesc,
                                                               const int32_t* __restrict__ gate) {
   if (gated_off(gate)) return;
@@ -165,16 +166,29 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
   // the chunk's B fragments: column 16w + li, k = 32 ks + 8 lq .. + 7.
   // Columns past n read column n - 1 and rows past m row m - 1 (their results
   // are never stored), so the chunk loads need no branch.
+  // BFRAG: B comes as bf16 MFMA fragments (the FragDesc layout of B, ldb
+  // holding its column-group count): one 16-byte load per k step, no rounding
+  using BV = std::conditional_t<BFRAG, bf16x8[NKS], f32x4[NKS][2]>;
   const HODLR_GLOBAL f32x4* zb = (const HODLR_GLOBAL f32x4*)zero_page<double>();
-  auto load_b = [&](f32x4 (&bv)[NKS][2], int c0) {
+  auto load_b = [&](BV& bv, int c0) {
+    if constexpr (BFRAG) {
+      const int ng = int(ldb);
+      const HODLR_GLOBAL bf16x8* fb =
+          (const HODLR_GLOBAL bf16x8*)glob((const bf16*)d.B) + int64_t(min(c0 / 16 + wave, ng - 1)) * 64 + lane;
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) bv[ks] = fb[int64_t(min(ks, nks - 1)) * ng * 64];
+      return;
+    }
     const int col = min(c0 + 16 * wave + li, n - 1);
     const HODLR_GLOBAL f32x4* bc = (const HODLR_GLOBAL f32x4*)(B + int64_t(col) * ldb);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       const int kk = 32 * ks + 8 * lq;
       const bool ok = ks < nks && kk + 8 <= ldb;
-      bv[ks][0] = *(ok ? bc + kk / 4 : zb);
-      bv[ks][1] = *(ok ? bc + kk / 4 + 1 : zb + 1);
+      if constexpr (!BFRAG) {
+        bv[ks][0] = *(ok ? bc + kk / 4 : zb);
+        bv[ks][1] = *(ok ? bc + kk / 4 + 1 : zb + 1);
+      }
     }
   };
   const HODLR_GLOBAL float* crow = Cin + min(r0 + fr, m - 1);
@@ -183,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
     for (int q = 0; q < CPT; ++q) cv[q] = crow[int64_t(min(c0 + fc + 4 * q, n - 1)) * ldi];
   };
   // one chunk: product, accumulators through LDS, C written coalesced
-  auto chunk = [&](int c0, const f32x4 (&bv)[NKS][2], const float (&cv)[CPT]) {
+  auto chunk = [&](int c0, const BV& bv, const float (&cv)[CPT]) {
     f32x4 acc[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x4{0, 0, 0, 0};
@@ -191,10 +205,14 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
     for (int ks = 0; ks < NKS; ++ks) {
       if (ks < nks) {
         bf16x8 b;
+        if constexpr (BFRAG) {
+          b = bv[ks];
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          b[e] = bf16(bv[ks][0][e]);
-          b[4 + e] = bf16(bv[ks][1][e]);
+          for (int e = 0; e < 4; ++e) {
+            b[e] = bf16(bv[ks][0][e]);
+            b[4 + e] = bf16(bv[ks][1][e]);
+          }
         }
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
@@ -215,7 +233,7 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
   };
   // two register sets, no copies between them: the next chunk's loads are
   // issued before a chunk's product and stay in flight through it
-  f32x4 bvA[NKS][2], bvB[NKS][2];
+  BV bvA, bvB;
   float cvA[CPT], cvB[CPT];
   load_b(bvA, 0);
   load_c(cvA, 0);
@@ -598,7 +616,9 @@ template <> __device__ __forceinline__ bf16 store_as<bf16>(double x) { return bf
 // 2^20 x 1280 V: 2.8 ms (VEC) against 4.0 ms for one fragment per wave of
 // 8-byte loads.
 template <bool VEC>
-__global__ __launch_bounds__(kThreads) void frag_bf16_kernel(const FragDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kThreads) void frag_bf16_kernel(const FragDesc* __restrict__ descs, int ndesc,
+                                                             const int32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   constexpr int FPW = kFragPerWave;
   const int64_t bid = blockIdx.x;
   const FragDesc d = descs[find_desc(descs, ndesc, bid)];
@@ -1895,11 +1915,12 @@ hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans, int role, int til
   return hipGetLastError();
 }
 
-hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
+hipError_t launch_panel_gemm(DT ts, DT mt, bool bfrag, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
                              const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   if (ts != F32 || mt != BF16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(panel_gemm_kernel, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
+  auto k = bfrag ? panel_gemm_kernel<true> : panel_gemm_kernel<false>;
+  hipLaunchKernelGGL(k, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
   return hipGetLastError();
 }
 
@@ -1916,10 +1937,10 @@ hipError_t launch_stream(bool trans, int ncols_max, bool vec, const StreamDesc* 
   return hipGetLastError();
 }
 
-hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, hipStream_t s) {
+hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
   auto k = vec ? frag_bf16_kernel<true> : frag_bf16_kernel<false>;
-  hipLaunchKernelGGL(k, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n);
+  hipLaunchKernelGGL(k, dim3(grid1(tiles)), dim3(kThreads), 0, s, d, n, gate);
   return hipGetLastError();
 }
 

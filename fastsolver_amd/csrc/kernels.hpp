@@ -32,16 +32,18 @@ constexpr int64_t kNormChunk = 65536;
 hipError_t launch_gemm(DT ta, DT tb, DT tc, DT mt, bool trans_a, int role, int tile, const GemmDesc* d, int n,
                        int64_t tiles, hipStream_t s, const int32_t* gate = nullptr);
 // row-panel GEMM (k <= 128, fp32 storage, bf16 MFMA): one workgroup per 64-row panel of C
-// sweeping all columns; B's leading dimension a multiple of 8 with zero rows past k;
+// sweeping all columns; B's leading dimension a multiple of 8 with zero rows past k, or with
+// `bfrag` B as bf16 MFMA fragments in the FragDesc layout of B (ldb = its column-group count);
 // descriptors tiled by 64-row panels
 constexpr int kPanelKMax = 128;
-hipError_t launch_panel_gemm(DT ts, DT mt, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
+hipError_t launch_panel_gemm(DT ts, DT mt, bool bfrag, const GemmDesc* d, int n, int64_t tiles, hipStream_t s,
                              const int32_t* gate = nullptr);
 // fp64 -> bf16 MFMA A fragments of the transpose (FragDesc), 4 kFragPerWave fragments per
 // workgroup (one column group, consecutive row blocks); `vec`: 32-byte loads, allowed when every
 // src is 32-byte aligned, every lds a multiple of 4 and every rows a multiple of 8
 constexpr int kFragPerWave = 4;
-hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, hipStream_t s);
+hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, hipStream_t s,
+                            const int32_t* gate = nullptr);
 // bf16 plans' reduce W = A^T B (GemmDesc, trans): A in the FragDesc fragment layout of A^T
 // (lda = its column-group count), B fp32, C fp64; tiles of kReduceRows x kReduceCols, split-K
 // chunks (kchunk) a multiple of 32 rows

@@ -329,6 +329,38 @@ struct Builder {
     sum(pr, cat == CAT_REDUCE ? CAT_REDUCE_SUM : cat, sums);
   }
 
+  // bf16 plans' broadcasts with eta as bf16 MFMA fragments (G::ldb = their
+  // column-group count): the row-panel kernel only, 64-row panels
+  void bcast_frag(Proc& pr, Cat cat, const std::vector<G>& gs) {
+    std::vector<GemmDesc> d;
+    double flops = 0, bytes = 0;
+    int64_t tiles = 0;
+    for (const G& g : gs) {
+      if (g.m <= 0 || g.n <= 0) continue;
+      GemmDesc x{};
+      x.A = g.A; x.B = g.B; x.Cin = g.Cin; x.Cout = g.Cout;
+      x.lda = g.lda; x.ldb = g.ldb; x.ldcin = g.ldcin; x.ldcout = g.ldcout;
+      x.m = g.m; x.n = g.n; x.k = g.k; x.kchunk = g.k; x.nsplit = 1;
+      x.alpha = g.alpha; x.beta = g.beta;
+      x.tiles_m = int(cdiv(g.m, 64));
+      x.tiles_n = 1;
+      x.tile_begin = tiles;
+      tiles += x.tiles_m;
+      d.push_back(x);
+      flops += 2.0 * g.m * g.n * g.k;
+      bytes += double(g.m) * g.k * 4 + double(g.k) * g.n * 2 + double(g.m) * g.n * 4 * (g.beta != 0.0 ? 2 : 1);
+    }
+    if (d.empty()) return;
+    Launch L;
+    L.kind = K_GEMM; L.cat = cat;
+    L.t[0] = L.t[1] = L.t[2] = F32; L.t[3] = BF16;
+    L.role = 2; L.tile = 3; L.alt[0] = L.alt[1] = 3; L.mode = 1;
+    L.count = int(d.size()); L.flops = flops; L.bytes = bytes;
+    L.off = L.off_t[3] = blob.add(d);
+    L.tiles = L.tiles_t[3] = tiles;
+    pr.l.push_back(L);
+  }
+
   void sum(Proc& pr, Cat cat, std::vector<SumDesc> v) {
     std::vector<SumDesc> d;
     int64_t st = 0;
@@ -819,7 +851,12 @@ int layout_plan(hodlr_plan_s* P, Layout& L) {
       const int64_t nd = t.col_beg(x.depth + 1);
       S.sbuf[i] = L.take(size_t(r) * r * 8);
       S.eta[i] = L.take(size_t(r) * std::max<int64_t>(nd, c) * 8);
-      if (we != 8) S.etaf[i] = L.take(2 * size_t(cdiv(r, 8) * 8) * std::max<int64_t>(nd, c) * we);
+      if (we != 8) {
+        // fp32 copies of both etas (ld r rounded up to 8) or, in bf16 plans, their bf16 MFMA fragments
+        const int64_t ncol = std::max<int64_t>(nd, c);
+        const size_t f32 = size_t(cdiv(r, 8) * 8) * ncol * we, frag = size_t(cdiv(r, 32) * 32) * cdiv(ncol, 16) * 16 * 2;
+        S.etaf[i] = L.take(2 * std::max(f32, frag));
+      }
       S.spiv[i] = L.take(size_t(r) * 4);
       S.sperm[i] = L.take(size_t(r) * 4);
       S.sdinv[i] = L.take(size_t(cdiv(r, 64)) * 2 * 64 * 64 * 8);
@@ -1027,6 +1064,16 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       std::vector<CopyDesc> sform, eround;
       std::vector<LuDesc> slu;
       std::vector<TrsmDesc> str;
+      std::vector<FragDesc> efrag;
+      // bf16 plans hand the factor's broadcast its etas as bf16 fragments when
+      // every node rank of the height fits the row-panel kernel (k <= kPanelKMax;
+      // measured 14.0 -> 12.1 ms at N = 2^20, c = 32, rank 128).  The
+      // correction solves' broadcasts (c columns) keep fp32 etas and the
+      // tunable choice between the row panel and the 64 x 32 engine tiles.
+      bool bfrag = P->mt == BF16 && mode == 0;
+      for (SysState& S : P->sys)
+        for (const TreeNode& Pn : S.tree.nodes)
+          if (!Pn.is_leaf() && Pn.height == h && S.tree.rank(Pn.depth + 1) > kPanelKMax) bfrag = false;
       for (SysState& S : P->sys) {
         const Tree& t = S.tree;
         const int64_t c = t.nrhs;
@@ -1100,7 +1147,20 @@ void build_all(hodlr_plan_s* P, Builder& B) {
           // workspace is cleared at creation and those rows are never written)
           const int rp = int(cdiv(r, 8) * 8);
           int ldeta = r;
-          if (wt != F64) {
+          if (bfrag) {
+            // bf16 plans: both etas as the bf16 MFMA fragments the row-panel
+            // broadcast loads directly (the rounding its products would do)
+            char* ef = P->ws.p + S.etaf[pi];
+            const size_t fbytes = size_t(cdiv(r, 32) * 32) * cdiv(nd, 16) * 16 * 2;
+            for (int k = 0; k < 2; ++k) {
+              FragDesc f{};
+              f.src = (const double*)etas[k]; f.lds = r; f.dst = ef + size_t(k) * fbytes;
+              f.rows = r; f.cols = nd; f.ngroups = int(cdiv(nd, 16));
+              efrag.push_back(f);
+              etas[k] = f.dst;
+            }
+            ldeta = int(cdiv(nd, 16));
+          } else if (wt != F64) {
             char* ef = P->ws.p + S.etaf[pi];
             for (int k = 0; k < 2; ++k) {
               CopyDesc cd{};
@@ -1139,7 +1199,9 @@ void build_all(hodlr_plan_s* P, Builder& B) {
       B.trsm(pn, ct, F64, F64, F64, str);
       B.gemm(pn, cg, F64, F64, F64, F64, false, 0, ng2);
       B.copy(pn, cg, F64, wt, eround);
-      B.gemm(pb, cb, wt, wt == F64 ? F64 : wt, wt, P->mt, false, 2, bc);
+      B.frag(pn, cg, efrag);
+      if (bfrag) B.bcast_frag(pb, cb, bc);
+      else B.gemm(pb, cb, wt, wt == F64 ? F64 : wt, wt, P->mt, false, 2, bc);
     }
   }
 
@@ -1211,7 +1273,7 @@ void build_all(hodlr_plan_s* P, Builder& B) {
     // every kind a sweep holds takes a gate; a launch that ignored it would
     // run in the sweeps the check has switched off
     if (L.kind != K_GEMM && L.kind != K_SUM && L.kind != K_COPY && L.kind != K_TRSM && L.kind != K_NORM &&
-        L.kind != K_NORM_FIN && L.kind != K_CHECK && L.kind != K_STREAM)
+        L.kind != K_NORM_FIN && L.kind != K_CHECK && L.kind != K_STREAM && L.kind != K_FRAG)
       B.bad_gate = true;
   }
 }
@@ -1230,7 +1292,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
         else if (L.gid < int(P->tile_choice.size()) && P->tile_choice[L.gid] >= 0) tile = L.alt[P->tile_choice[L.gid]];
       }
       const GemmDesc* gd = (const GemmDesc*)(P->descs.p + L.off_t[tile]);
-      if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], gd, L.count, L.tiles_t[3], s, gate);
+      if (tile == 3) return launch_panel_gemm(L.t[0], L.t[3], L.mode != 0, gd, L.count, L.tiles_t[3], s, gate);
       if (tile == 4) return launch_reduce_bf16(gd, L.count, L.tiles_t[4], s, gate);
       return launch_gemm(L.t[0], L.t[1], L.t[2], L.t[3], L.trans, L.role, tile, gd, L.count, L.tiles_t[tile], s,
                          gate);
@@ -1241,7 +1303,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
     case K_STREAM: return launch_stream(L.trans, L.nmax, L.mode != 0, (const StreamDesc*)d, L.count, L.tiles, s, gate);
-    case K_FRAG: return launch_frag_bf16(L.mode != 0, (const FragDesc*)d, L.count, L.tiles, s);
+    case K_FRAG: return launch_frag_bf16(L.mode != 0, (const FragDesc*)d, L.count, L.tiles, s, gate);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
     case K_LU_PERM: return launch_lu_perm((const LuDesc*)d, L.count, s);
