@@ -721,6 +721,14 @@ constexpr int kPanelThreads = 4 * kPanelNT;   // largest block the panel kernel 
 
 // sub-panel width: a[RPT][PW] stays within 128 registers
 template <class T, int RPT> struct PanelWidth { static constexpr int value = sizeof(T) == 4 && RPT < 4 ? 32 : 16; };
+// waves per SIMD the compiler is held to: fp32 panels of 769-1024-row blocks
+// fit 128 registers (a few spills), so four blocks share a CU instead of two
+// (the panels of a batch of 1024 leaves of 1024 rows run in one round:
+// 5.7 -> 4.7 ms per factorization); the other variants keep the compiler's
+// register choice (fp64 at 168 registers spills heavily)
+template <class T, int NT, int RPT> struct PanelOcc {
+  static constexpr int value = sizeof(T) == 4 && NT == 256 && RPT == 4 ? 4 : 1;
+};
 
 template <class T>
 __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
@@ -775,8 +783,9 @@ __device__ __forceinline__ void swap_rows_in_column(PT A, int64_t col, int r1, i
   }
 }
 
-template <class T, int NT, int RPT>
-__global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__ descs, int k0) {
+template <class T, int NT, int RPT, int OCC = 1>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC))) void lu_panel_kernel(
+    const LuDesc* __restrict__ descs, int k0) {
   constexpr int NW = NT / 64;
   constexpr int PW = PanelWidth<T, RPT>::value;
   constexpr int NSP = kLuNB / PW;
@@ -985,17 +994,26 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
   if (tid == 0 && s_info != 0 && *d.info == 0) *d.info = s_info;
 }
 
-// Whole LU with partial pivoting of blocks of at most kSmallLU = 128 rows
-// (the node systems S, small leaves), one 1024-thread workgroup per block,
-// the block in registers: thread (ty, tx) = (t / 32, t % 32) holds
-// A(ty + 32u, tx + 32w), u, w < 4 (zero past m), so a column step costs each
-// thread at most 16 FMAs.  Column step j (its column block v = j / 32 is a
+// f(integral_constant<I>) for I = B .. E-1, each call with a compile-time index
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>());
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Whole LU with partial pivoting of blocks of at most G * NU rows (the node
+// systems S, small leaves), one G x G-thread workgroup per block (G = 8, 16
+// or 32), the block in registers: thread (ty, tx) = (t / G, t % G) holds
+// A(ty + Gu, tx + Gw), u, w < NU (zero past m), so a column step costs each
+// thread at most NU^2 FMAs.  Column step j (its column block v = j / G is a
 // compile-time index, so blocks above / left of the step are skipped
 // statically):
 //   1. the owners of column j post their best candidate (first largest |a|
-//      at or below row j: idamax, NaN never wins); lanes 0..31 of every wave
-//      read the 32 candidates and combine them with DPP row shifts and a row
-//      broadcast;
+//      at or below row j: idamax, NaN never wins); lanes 0..G-1 of every
+//      wave read the G candidates and combine them with DPP row shifts (and,
+//      for G = 32, a row broadcast);
 //   2. the owners of rows j and p post them; the exchange is done in
 //      registers; the owners of column j scale it by the pivot's reciprocal
 //      (dgetf2's dscal; unscaled below a zero pivot, whose column info
@@ -1006,19 +1024,20 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(const LuDesc* __restrict__
 // Three LDS-only barriers per column, no global memory between the load and
 // the store of the block.  Writes LU in place, ipiv and info; the inverted
 // diagonal blocks and perm come from lu_dinv_kernel and lu_perm_kernel.
-template <class T>
-__global__ __launch_bounds__(1024) void lu_small_kernel(const LuDesc* __restrict__ descs) {
-  constexpr int G = 32, NU = kSmallLU / G;   // thread grid side, blocks per thread side
+template <class T, int G, int NU>
+__global__ __launch_bounds__(G * G) void lu_small_kernel(const LuDesc* __restrict__ descs) {
+  static_assert(G == 8 || G == 16 || G == 32, "thread grid side");
+  constexpr int MAXM = G * NU;   // largest block
   const LuDesc d = descs[blockIdx.x];
   const int m = d.m;
   if (m <= 0) return;
   HODLR_GLOBAL T* A = glob((T*)d.A);
   const int64_t lda = d.lda;
   const int tid = threadIdx.x, tx = tid % G, ty = tid / G, lane = tid & 63;
-  __shared__ T s_rj[kSmallLU], s_rp[kSmallLU], s_l[kSmallLU];
+  __shared__ T s_rj[MAXM], s_rp[MAXM], s_l[MAXM];
   __shared__ T s_cv[G];
   __shared__ int s_ci[G];
-  __shared__ int s_ipiv[kSmallLU];
+  __shared__ int s_ipiv[MAXM];
   T a[NU][NU];
 #pragma unroll
   for (int u = 0; u < NU; ++u)
@@ -1048,20 +1067,23 @@ __global__ __launch_bounds__(1024) void lu_small_kernel(const LuDesc* __restrict
         s_ci[ty] = bi;
       }
       lds_barrier();
-      T pv = s_cv[lane & 31];
-      int p = s_ci[lane & 31];
+      T pv = s_cv[lane & (G - 1)];
+      int p = s_ci[lane & (G - 1)];
       argmax_step<T, 0x111, 0xf>(pv, p);   // row_shr:1
       argmax_step<T, 0x112, 0xf>(pv, p);   // row_shr:2
-      argmax_step<T, 0x114, 0xf>(pv, p);   // row_shr:4
-      argmax_step<T, 0x118, 0xf>(pv, p);   // row_shr:8: lane 15 of each 16-lane row holds its best
-      argmax_step<T, 0x142, 0xa>(pv, p);   // row_bcast:15 into rows 1 and 3: lane 31 holds the best of 32
-      p = __builtin_amdgcn_readlane(p, 31);
+      argmax_step<T, 0x114, 0xf>(pv, p);   // row_shr:4: lane 7 holds the best of lanes 0..7
+      // row_shr:8: lane 15 of each 16-lane row holds its best
+      if constexpr (G >= 16) argmax_step<T, 0x118, 0xf>(pv, p);
+      // row_bcast:15 into rows 1 and 3: lane 31 holds the best of 32
+      if constexpr (G == 32) argmax_step<T, 0x142, 0xa>(pv, p);
+      p = __builtin_amdgcn_readlane(p, G - 1);
       if constexpr (sizeof(T) == 8) {
         const long long bits = __double_as_longlong(pv);
-        const int lo = __builtin_amdgcn_readlane(int(bits), 31), hi = __builtin_amdgcn_readlane(int(bits >> 32), 31);
+        const int lo = __builtin_amdgcn_readlane(int(bits), G - 1);
+        const int hi = __builtin_amdgcn_readlane(int(bits >> 32), G - 1);
         pv = __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
       } else {
-        pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 31));
+        pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), G - 1));
       }
       if (p >= m) p = j;   // no candidate (all NaN): keep row j
       if (!(pv > T(0)) && info == 0) info = j + 1;
@@ -1125,10 +1147,7 @@ __global__ __launch_bounds__(1024) void lu_small_kernel(const LuDesc* __restrict
       }
     }
   };
-  block(std::integral_constant<int, 0>());
-  block(std::integral_constant<int, 1>());
-  block(std::integral_constant<int, 2>());
-  block(std::integral_constant<int, 3>());
+  static_for<0, NU>(block);
 #pragma unroll
   for (int u = 0; u < NU; ++u)
 #pragma unroll
@@ -1136,7 +1155,7 @@ __global__ __launch_bounds__(1024) void lu_small_kernel(const LuDesc* __restrict
       const int r = ty + G * u, c = tx + G * w;
       if (r < m && c < m) A[r + int64_t(c) * lda] = a[u][w];
     }
-  for (int j = tid; j < m; j += 1024) d.ipiv[j] = s_ipiv[j];   // (the last barrier ordered s_ipiv)
+  for (int j = tid; j < m; j += G * G) d.ipiv[j] = s_ipiv[j];   // (the last barrier ordered s_ipiv)
   if (tid == 0) *d.info = info;
 }
 
@@ -1975,7 +1994,8 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
 #define HODLR_PANEL(NT, RPT)                                                                         \
   do {                                                                                               \
     if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT, RPT>), g, dim3(NT), 0, s, d, k0); \
-    else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT>), g, dim3(NT), 0, s, d, k0);           \
+    else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT, PanelOcc<float, NT, RPT>::value>), g,  \
+                            dim3(NT), 0, s, d, k0);                                                  \
   } while (0)
   if (rows_max <= 4 * 64) {
     // short panels: one wave per block (no cross-wave pivot combine, and four
@@ -1998,10 +2018,18 @@ hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_lu_small(DT t, const LuDesc* d, int n, hipStream_t s) {
+hipError_t launch_lu_small(DT t, int mmax, const LuDesc* d, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (t == F64) hipLaunchKernelGGL(lu_small_kernel<double>, dim3(n), dim3(1024), 0, s, d);
-  else hipLaunchKernelGGL(lu_small_kernel<float>, dim3(n), dim3(1024), 0, s, d);
+  if (mmax > kSmallLU) return hipErrorInvalidValue;
+  if (mmax <= 64) {
+    // one wave per block: 8 x 8 threads, 8 x 8 entries each
+    if (t == F64) hipLaunchKernelGGL((lu_small_kernel<double, 8, 8>), dim3(n), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL((lu_small_kernel<float, 8, 8>), dim3(n), dim3(64), 0, s, d);
+  } else {
+    constexpr int G = kSmallLUGrid, NU = kSmallLU / kSmallLUGrid;
+    if (t == F64) hipLaunchKernelGGL((lu_small_kernel<double, G, NU>), dim3(n), dim3(G * G), 0, s, d);
+    else hipLaunchKernelGGL((lu_small_kernel<float, G, NU>), dim3(n), dim3(G * G), 0, s, d);
+  }
   return hipGetLastError();
 }
 

@@ -65,7 +65,12 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
 hipError_t launch_lu_perm(const LuDesc* d, int n, hipStream_t s);
 // whole LU (in place, ipiv, info; not dinv / perm) of blocks of at most kSmallLU rows, in registers
 constexpr int kSmallLU = 128;
-hipError_t launch_lu_small(DT t, const LuDesc* d, int n, hipStream_t s);
+// thread grid side of the register-resident LU for 65-128 rows: 16 x 16
+// threads, 8 x 8 entries each (32 x 32 threads, 16 waves per block, measured
+// slower: more waves per SIMD repeat the per-column pivot search and
+// exchange); blocks of at most 64 rows take one wave (8 x 8 threads)
+constexpr int kSmallLUGrid = 16;
+hipError_t launch_lu_small(DT t, int mmax, const LuDesc* d, int n, hipStream_t s);
 // inverted 64 x 64 diagonal blocks b0 .. b0+nblk-1 of every factor
 hipError_t launch_lu_dinv(DT t, const LuDesc* d, int n, int b0, int nblk, hipStream_t s);
 hipError_t launch_lu_init(const LuDesc* d, int n, hipStream_t s);   // info = 0

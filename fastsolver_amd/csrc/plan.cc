@@ -403,14 +403,14 @@ struct Builder {
     int mmax = 0;
     for (auto& x : v) mmax = std::max(mmax, x.m);
     size_t off = blob.add(v);
-    if (mmax <= kSmallLU && mmax > kLuNB) {
+    if (mmax <= kSmallLU) {
       // every block in registers: one launch for the whole factorization
-      // (info included), then the inverted diagonal blocks and perm.  Blocks
-      // of at most 64 rows keep the one-wave panel path: a 1024-thread group
-      // per block would leave large batches of them (many small systems)
-      // with one block in flight per CU
+      // (info included), then the inverted diagonal blocks and perm; a
+      // 256-thread workgroup per block of 65-128 rows, one wave per block of
+      // at most 64 (batches of thousands of them -- many small systems --
+      // keep many blocks in flight per CU)
       Launch S;
-      S.kind = K_LU_SMALL; S.cat = panel_cat; S.t[0] = t; S.off = off; S.count = int(v.size());
+      S.kind = K_LU_SMALL; S.cat = panel_cat; S.t[0] = t; S.off = off; S.count = int(v.size()); S.mmax = mmax;
       for (auto& x : v) S.flops += 2.0 / 3.0 * double(x.m) * x.m * x.m;
       pr.l.push_back(S);
       Launch Dv;
@@ -1298,7 +1298,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
                          gate);
     }
     case K_LU_DINV: return launch_lu_dinv(L.t[0], (const LuDesc*)d, L.count, L.k0 / kLuNB, std::max(1, L.mode), s);
-    case K_LU_SMALL: return launch_lu_small(L.t[0], (const LuDesc*)d, L.count, s);
+    case K_LU_SMALL: return launch_lu_small(L.t[0], L.mmax, (const LuDesc*)d, L.count, s);
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s, gate);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);

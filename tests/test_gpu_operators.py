@@ -113,7 +113,7 @@ def test_bf16_gemm_integer_known_answer_and_rounding():
         assert np.abs(dC.download() - A.astype(np.float64) @ B.astype(np.float64)).max() > 1e-4
 
 
-@pytest.mark.parametrize("n", [64, 100, 128, 130, 200, 513, 777, 1024])
+@pytest.mark.parametrize("n", [5, 17, 64, 100, 128, 130, 200, 513, 777, 1024])
 def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     """The in-place LU factors equal LAPACK getrf's: the pivot search picks the
     same rows (first largest |a|, dgetf2's idamax), so L and U agree to rounding
@@ -129,11 +129,12 @@ def test_dgesv_batched_lu_factors_and_pivots_match_lapack(n):
     assert np.abs(got - lu).max() <= 1e-10 * np.abs(lu).max()
 
 
-@pytest.mark.parametrize("n,zero_col", [(100, 70), (128, 0), (72, 71)])
+@pytest.mark.parametrize("n,zero_col", [(100, 70), (128, 0), (72, 71), (64, 10), (40, 39), (9, 0)])
 def test_dgesv_batched_zero_pivot_matches_lapack_getrf(n, zero_col):
-    """Blocks of 65-128 rows take the register-resident LU: a zero column
-    gives an exactly zero pivot there; info and the factors (the column left
-    unscaled, the elimination continued) equal LAPACK getrf's."""
+    """Blocks of at most 128 rows take the register-resident LU (one wave per
+    block up to 64 rows, 256 threads above): a zero column gives an exactly
+    zero pivot there; info and the factors (the column left unscaled, the
+    elimination continued) equal LAPACK getrf's."""
     import scipy.linalg.lapack as lapack
     rng = np.random.default_rng(300 + n)
     A = np.asfortranarray(rng.standard_normal((n, n)))
