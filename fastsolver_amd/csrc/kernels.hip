@@ -724,11 +724,14 @@ template <class T, int RPT> struct PanelWidth { static constexpr int value = siz
 // waves per SIMD the compiler is held to: fp32 panels of 769-1024-row blocks
 // fit 128 registers (a few spills), so four blocks share a CU instead of two
 // (the panels of a batch of 1024 leaves of 1024 rows run in one round:
-// 5.7 -> 4.7 ms per factorization); the other variants keep the compiler's
-// register choice (fp64 at 168 registers spills heavily)
+// 5.7 -> 4.7 ms per factorization); used for batches of more blocks than two
+// per CU (256 blocks, one per CU, ran 5 % slower with the spills); the other
+// variants keep the compiler's register choice (fp64 at 168 registers spills
+// heavily)
 template <class T, int NT, int RPT> struct PanelOcc {
   static constexpr int value = sizeof(T) == 4 && NT == 256 && RPT == 4 ? 4 : 1;
 };
+constexpr int kPanelOccBlocks = 512;   // two blocks per CU on 256 CUs
 
 template <class T>
 __device__ __forceinline__ void pivot_combine(T& bv, int& bi, T ov, int oi) {
@@ -1994,8 +1997,10 @@ hipError_t launch_lu_panel(DT t, const LuDesc* d, int n, int k0, int rows_max, h
 #define HODLR_PANEL(NT, RPT)                                                                         \
   do {                                                                                               \
     if (t == F64) hipLaunchKernelGGL((lu_panel_kernel<double, NT, RPT>), g, dim3(NT), 0, s, d, k0); \
-    else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT, PanelOcc<float, NT, RPT>::value>), g,  \
-                            dim3(NT), 0, s, d, k0);                                                  \
+    else if (n > kPanelOccBlocks)                                                                    \
+      hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT, PanelOcc<float, NT, RPT>::value>), g,      \
+                         dim3(NT), 0, s, d, k0);                                                     \
+    else hipLaunchKernelGGL((lu_panel_kernel<float, NT, RPT>), g, dim3(NT), 0, s, d, k0);           \
   } while (0)
   if (rows_max <= 4 * 64) {
     // short panels: one wave per block (no cross-wave pivot combine, and four
