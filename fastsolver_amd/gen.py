@@ -170,3 +170,10 @@ def single_launch_case():
 def accuracy_case():
     """test/main.cc:174-185,257-259: rank 6, N = 480, threshold 15, 2 rhs, diag 10, seed 1123."""
     return SystemSpec(15 * 32, 15, 2, [6], kind="circulant", seed=1123, diag=10.0)
+
+
+def test3_1_case():
+    """test/main.cc:157-171,245-281 (test 3.1, the reference's timed run): rank 300,
+    N = 400 * 2^8 = 102,400, threshold 600 (leaves of 400, 8 levels), 2 rhs,
+    diag 1e5, seed 1123."""
+    return SystemSpec(400 * (1 << 8), 600, 2, [300], kind="circulant", seed=1123, diag=1.0e5)

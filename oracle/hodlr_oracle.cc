@@ -505,16 +505,20 @@ int oracle_dense_solve_ld(int n, int nrhs, const double* A, int lda, const doubl
 
 // Woodbury closed form for the reference's test matrix A = diag*I + U U^T with
 // U(i,j) = (i+j) mod r (direct_solve.cc:168-189): x = (b - U (diag*I + U^T U)^-1 U^T b) / diag,
-// evaluated in long double.  O(N r^2).
+// evaluated in long double.  O(r^3 + N r): U's row i depends on i mod r only.
 int oracle_woodbury_circulant(int64_t N, int r, double diag, int nrhs, const double* B, int64_t ldb,
                               double* Xo, int64_t ldxo) {
-  // G = U^T U (r x r): entries are exact integers
+  // G = U^T U (r x r) = sum over residues p of cnt[p] * u_p u_p^T, u_p(a) =
+  // (p + a) mod r, cnt[p] = #{0 <= i < N : i mod r = p}: entries are exact
+  // integers (below N r^2 < 2^64, so every partial sum is exact in long double)
   std::vector<long double> G(size_t(r) * r, 0.0L);
-  for (int64_t i = 0; i < N; ++i)
+  for (int p = 0; p < r && p < N; ++p) {
+    const long double cnt = (long double)((N - 1 - p) / r + 1);
     for (int a = 0; a < r; ++a) {
-      long double ua = (long double)((i + a) % r);
-      for (int b = 0; b < r; ++b) G[a + size_t(b) * r] += ua * (long double)((i + b) % r);
+      const long double ua = cnt * (long double)((p + a) % r);
+      for (int b = 0; b < r; ++b) G[a + size_t(b) * r] += ua * (long double)((p + b) % r);
     }
+  }
   for (int a = 0; a < r; ++a) G[a + size_t(a) * r] += diag;
   for (int j = 0; j < nrhs; ++j) {
     std::vector<long double> z(r, 0.0L);

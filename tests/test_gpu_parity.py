@@ -303,6 +303,41 @@ def test_cfg3_full_size_mixed(prec):
     assert rel(ds.solution(), Xo[:, :spec.nrhs]) <= TOL
 
 
+def test_reference_test3_circulant_full_size():
+    """The reference's own timed workload, test 3.1 (test/main.cc:157-171): its
+    circulant matrix diag*I + U U^T, U(i,j) = (i+j) mod r, at full size (N =
+    102,400, rank 300, leaves of 400, 8 levels, 2 rhs, diag 1e5), the solution
+    against the Woodbury closed form, as direct_solve.cc:168-225 checks it.
+    The matrix has kappa = 1 + lambda_max(U^T U) / 1e5 = 6.9e6: the FP64 pass
+    reaches a relative residual of 1.4e-11 and the FP64 refinement takes it
+    below 1e-12; an FP32 or BF16 factorization (kappa * u >= 0.4) cannot
+    contract the refinement, and hodlr_solve reports that instead of a
+    solution."""
+    from fastsolver_amd._lib import HodlrError
+    spec = gen.test3_1_case()
+    nl, md = gen.tree_info(spec)[:2]
+    assert (nl, md) == (256, 8)
+    X, V, K = host_system(spec)
+    xw = oracle.woodbury_circulant(spec.n, spec.ranks[0], spec.diag, X[:, :spec.nrhs].copy(order="F"))
+    ds = hodlr.DeviceSystem.from_host(spec, X, V, K)
+    del X, V, K
+    plan = hodlr.Plan([ds], hodlr.PREC_FP64)
+    rep = plan.solve(tol=TOL, maxiter=10)
+    plan.synchronize()
+    e = rel(ds.solution(), xw)
+    print(f"  FP64: first {rep.relres_first:.2e}, {rep.iterations} sweeps, relres {rep.relres:.2e}, "
+          f"err vs Woodbury {e:.2e}", flush=True)
+    assert rep.converged and rep.relres <= TOL
+    assert e <= TOL
+    plan.close()
+    for prec in (hodlr.PREC_FP32, hodlr.PREC_BF16):
+        plan = hodlr.Plan([ds], prec)
+        with pytest.raises(HodlrError) as ei:
+            plan.solve(tol=TOL, maxiter=4)
+        assert ei.value.code == -7   # HODLR_ERR_NOTCONVERGED
+        plan.close()
+
+
 @pytest.mark.slow
 def test_cfg5_full_size_against_oracle():
     """N = 2^20, 10 levels, rank 128, 32 rhs at full size: every precision reaches

@@ -87,6 +87,20 @@ def test_oracle_reference_cases_vs_woodbury(name, case):
     assert _rel(Xo[:, :spec.nrhs], fx["x_ref"]) < 1e-13
 
 
+@pytest.mark.parametrize("n,r,diag", [(480, 30, 1.0e4), (197, 7, 10.0), (1000, 64, 1.0e5), (5, 9, 3.0)])
+def test_woodbury_closed_form_vs_dense_lapack(n, r, diag):
+    """The Woodbury closed form (its Gram matrix summed per residue i mod r)
+    against LAPACK on the assembled diag*I + U U^T, U(i,j) = (i+j) mod r,
+    N a multiple of r or not, and N < r."""
+    rng = np.random.default_rng(n + r)
+    b = np.asfortranarray(rng.standard_normal((n, 3)))
+    i = np.arange(n)[:, None]
+    U = ((i + np.arange(r)[None, :]) % r).astype(np.float64)
+    xd = np.linalg.solve(diag * np.eye(n) + U @ U.T, b)
+    xw = oracle.woodbury_circulant(n, r, diag, b)
+    assert _rel(xw, xd) < 1e-11 * max(1.0, np.linalg.cond(diag * np.eye(n) + U @ U.T) / 1e3)
+
+
 @pytest.mark.parametrize("name,cfg,scale", [("cfg1_n2048", 1, 8), ("cfg1_n16384", 1, 1)])
 def test_oracle_random_family_vs_dense_lapack(name, cfg, scale):
     spec = gen.config(cfg, scale=scale)[0]
