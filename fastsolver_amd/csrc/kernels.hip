@@ -723,7 +723,7 @@ constexpr int kPanelThreads = 4 * kPanelNT;   // largest block the panel kernel 
 template <class T, int RPT> struct PanelWidth { static constexpr int value = sizeof(T) == 4 && RPT < 4 ? 32 : 16; };
 // waves per SIMD the compiler is held to: fp32 panels of 769-1024-row blocks
 // fit 128 registers (a few spills), so four blocks share a CU instead of two
-// (the panels of a batch of 1024 leaves of 1024 rows run in one round:
+// (the panels of a batch of 1024 leaves of 1024 rows are all resident at once:
 // 5.7 -> 4.7 ms per factorization); used for batches of more blocks than two
 // per CU (256 blocks, one per CU, ran 5 % slower with the spills); the other
 // variants keep the compiler's register choice (fp64 at 168 registers spills
