@@ -99,10 +99,12 @@ struct NormDesc {
   int64_t tile_begin;
 };
 
-// FP64 streaming product with few columns (the residual and matvec sweeps):
+// FP64 product with few columns (the residual and matvec sweeps):
 //   C (m x n) = alpha * (op(A1) B1 + A2 B2) + beta * Cin
 // op(A1) = A1 (m x k1) or A1^T (A1: k1 x m); A2 (m x k2, never transposed)
-// may be absent (k2 = 0).  One workgroup per 256 rows x 16 columns of C.
+// may be absent (k2 = 0).  The streaming kernel takes op(A1) = A1, one
+// workgroup per 256 rows x 16 or 32 columns of C; transposed products go to
+// the tile engine.
 struct StreamDesc {
   const double* A1;
   const double* B1;

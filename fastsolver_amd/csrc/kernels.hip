@@ -252,10 +252,11 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
 }
 
 // ============================================================ FP64 streaming
-// C = alpha (op(A1) B1 + A2 B2) + beta Cin for products with few columns
-// (16 or 32 per tile; 16 for op(A1) = A1^T): the residual r = b - (blockdiag(K)
-// x + U Z) as one two-segment product per leaf row block, and V^T x of the HODLR matvec
-// (src/solver/gemm.cc:315-384 and :417-478 on c columns, in FP64).  At
+// C = alpha (A1 B1 + A2 B2) + beta Cin for products with few columns (16 or
+// 32 per tile): the residual r = b - (blockdiag(K) x + U Z) as one
+// two-segment product per leaf row block (src/solver/gemm.cc:417-478 on c
+// columns, in FP64; the matvec's V^T x runs on the tile engine, whose LDS
+// tiles read V's columns faster than per-lane transposed loads do).  At
 // c <= 32 these carry at most 8 flop per byte of A, under the FP64 ridge
 // (~10), so every wave owns 64 rows of C and the tile's columns and streams
 // its rows of A once, straight from global memory into FP64 MFMA operand
@@ -263,22 +264,21 @@ __global__ __launch_bounds__(kThreads) void panel_gemm_kernel(const GemmDesc* __
 // in flight while a step computes.  In the e-th MFMA of a step, the k slot q
 // of lane (p, q) = (l % 16, l / 16) carries k0 + 4q + e, so a lane's B values
 // are 4 consecutive rows of one column:
-//   NN: a(b, e) = A(r0 + 16b + p, k0 + 4q + e)   (16 rows of a column per 16 lanes)
+//   A:  a(b, e) = A(r0 + 16b + p, k0 + 4q + e)   (16 rows of a column per 16 lanes)
 //       VEC: a(b, e) = A(r0 + 4p + b, k0 + 4q + e)   (a lane's 4 blocks are 4
 //       consecutive rows: one 32-byte load, 512 contiguous bytes per 16 lanes)
-//   TN: a(b, e) = A(k0 + 4q + e, r0 + 16b + p)   (row i of C = column i of A)
 //   B:  b(c, e) = B(k0 + 4q + e, 16c + p)
 // VEC loads a lane's 4 consecutive values as one 32-byte vector (the host
 // selects it when the operands are 32-byte aligned, their leading dimensions
-// multiples of 4 and, for NN, m a multiple of 4).  Accumulator entry v of
-// lane (p, q), block (b, c): C row 16b + q + 4v (VEC NN: 4(q + 4v) + b),
-// column 16c + p.  The tile goes out through LDS, coalesced.
-template <bool TRANS, int NCB, bool VEC>
+// and m multiples of 4).  Accumulator entry v of lane (p, q), block (b, c):
+// C row 16b + q + 4v (VEC: 4(q + 4v) + b), column 16c + p.  The tile goes out
+// through LDS, coalesced.
+template <int NCB, bool VEC>
 __device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR_GLOBAL double* A, int64_t lda,
                                                const HODLR_GLOBAL double* B, int64_t ldb, int k, int r0, int m,
                                                int c0, int n) {
   constexpr int NB = 4, KS = 16;
-  constexpr bool QUAD = VEC && !TRANS;
+  constexpr bool QUAD = VEC;
   if (k <= 0) return;
   const int lane = threadIdx.x & 63, p = lane & 15, q = lane >> 4;
   // Rows past m and columns past n read the last row (quad) / column instead
@@ -292,7 +292,7 @@ __device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int row = min(r0 + 16 * b + p, m - 1);
-      arow[b] = TRANS ? A + int64_t(row) * lda + 4 * q : A + row + int64_t(4 * q) * lda;
+      arow[b] = A + row + int64_t(4 * q) * lda;
     }
   }
   const HODLR_GLOBAL double* bcol[NCB];
@@ -301,25 +301,16 @@ __device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR
   const HODLR_GLOBAL double* zero = zero_page<double>();
   // a full k step: no predicate, no clamp
   auto load = [&](double (&a)[NB][4], double (&bb)[NCB][4], int k0) {
-    if (VEC && TRANS) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const f64x4 t = *(const HODLR_GLOBAL f64x4*)(arow[b] + k0);
+    for (int e = 0; e < 4; ++e) {
+      const int64_t off = int64_t(k0 + e) * lda;   // wave-uniform
+      if (QUAD) {
+        const f64x4 t = *(const HODLR_GLOBAL f64x4*)(arow[0] + off);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) a[b][e] = t[e];
-      }
-    } else {
+        for (int b = 0; b < NB; ++b) a[b][e] = t[b];
+      } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t off = TRANS ? int64_t(k0 + e) : int64_t(k0 + e) * lda;   // wave-uniform
-        if (QUAD) {
-          const f64x4 t = *(const HODLR_GLOBAL f64x4*)(arow[0] + off);
-#pragma unroll
-          for (int b = 0; b < NB; ++b) a[b][e] = t[b];
-        } else {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) a[b][e] = arow[b][off];
-        }
+        for (int b = 0; b < NB; ++b) a[b][e] = arow[b][off];
       }
     }
 #pragma unroll
@@ -341,7 +332,7 @@ __device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR
       const int kk = k0 + 4 * q + e;
       const bool kok = kk < k;
       const int dk = (kok ? kk : k - 1) - 4 * q;
-      const int64_t off = TRANS ? int64_t(dk) : int64_t(dk) * lda;
+      const int64_t off = int64_t(dk) * lda;
 #pragma unroll
       for (int b = 0; b < NB; ++b) a[b][e] = QUAD ? arow[0][off + b] : arow[b][off];
 #pragma unroll
@@ -389,7 +380,7 @@ __device__ __forceinline__ void stream_segment(f64x4 (&acc)[4][NCB], const HODLR
   }
 }
 
-template <bool TRANS, int NCB, bool VEC>
+template <int NCB, bool VEC>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void stream_kernel(
     const StreamDesc* __restrict__ descs, int ndesc, const int32_t* __restrict__ gate) {
   if (gated_off(gate)) return;
@@ -409,8 +400,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
     for (int c = 0; c < NCB; ++c) acc[b][c] = f64x4{0, 0, 0, 0};
   if (r0 < m) {
-    stream_segment<TRANS, NCB, VEC>(acc, glob(d.A1), d.lda1, glob(d.B1), d.ldb1, d.k1, r0, m, c0, n);
-    if (d.k2 > 0) stream_segment<false, NCB, VEC>(acc, glob(d.A2), d.lda2, glob(d.B2), d.ldb2, d.k2, r0, m, c0, n);
+    stream_segment<NCB, VEC>(acc, glob(d.A1), d.lda1, glob(d.B1), d.ldb1, d.k1, r0, m, c0, n);
+    if (d.k2 > 0) stream_segment<NCB, VEC>(acc, glob(d.A2), d.lda2, glob(d.B2), d.ldb2, d.k2, r0, m, c0, n);
   }
   // epilogue: each 16-column block of the wave's 64-row tile through the
   // wave's LDS slice, then one row per lane, column by column (coalesced)
@@ -429,7 +420,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sc[p * PC + (VEC && !TRANS ? 4 * (q + 4 * v) + b : 16 * b + q + 4 * v)] = acc[b][c][v];
+      for (int v = 0; v < 4; ++v) sc[p * PC + (VEC ? 4 * (q + 4 * v) + b : 16 * b + q + 4 * v)] = acc[b][c][v];
     __syncthreads();
     if (row < m) {
 #pragma unroll
@@ -1946,15 +1937,14 @@ hipError_t launch_panel_gemm(DT ts, DT mt, bool bfrag, const GemmDesc* d, int n,
   return hipGetLastError();
 }
 
-hipError_t launch_stream(bool trans, int ncols_max, bool vec, const StreamDesc* d, int n, int64_t tiles,
-                         hipStream_t s, const int32_t* gate) {
+hipError_t launch_stream(int ncols_max, bool vec, const StreamDesc* d, int n, int64_t tiles, hipStream_t s,
+                         const int32_t* gate) {
   if (n <= 0 || tiles <= 0) return hipSuccess;
-  if (ncols_max > (trans ? 16 : kStreamCols)) return hipErrorInvalidValue;
+  if (ncols_max > kStreamCols) return hipErrorInvalidValue;
   dim3 g(grid1(tiles)), b(kThreads);
   const bool two = ncols_max > 16;
-  auto k = trans ? (vec ? stream_kernel<true, 1, true> : stream_kernel<true, 1, false>)
-                 : (two ? (vec ? stream_kernel<false, 2, true> : stream_kernel<false, 2, false>)
-                        : (vec ? stream_kernel<false, 1, true> : stream_kernel<false, 1, false>));
+  auto k = two ? (vec ? stream_kernel<2, true> : stream_kernel<2, false>)
+               : (vec ? stream_kernel<1, true> : stream_kernel<1, false>);
   hipLaunchKernelGGL(k, g, b, 0, s, d, n, gate);
   return hipGetLastError();
 }

@@ -49,14 +49,14 @@ hipError_t launch_frag_bf16(bool vec, const FragDesc* d, int n, int64_t tiles, h
 // chunks (kchunk) a multiple of 32 rows
 constexpr int kReduceRows = 128, kReduceCols = 32;
 hipError_t launch_reduce_bf16(const GemmDesc* d, int n, int64_t tiles, hipStream_t s, const int32_t* gate = nullptr);
-// FP64 streaming products (StreamDesc), tiles of kStreamRows rows x stream_tile_cols(ncols_max)
-// columns; `ncols_max` = the widest product of the batch (at most kStreamCols, 16 with `trans`); `vec`: 32-byte
-// vector loads, allowed when every A and B pointer is 32-byte aligned, every leading dimension a
-// multiple of 4 and, without `trans`, every m a multiple of 4
+// FP64 streaming products (StreamDesc: C = alpha (A1 B1 + A2 B2) + beta Cin), tiles of kStreamRows
+// rows x stream_tile_cols(ncols_max) columns; `ncols_max` = the widest product of the batch (at most
+// kStreamCols); `vec`: 32-byte vector loads, allowed when every A and B pointer is 32-byte aligned,
+// every leading dimension and every m a multiple of 4
 constexpr int kStreamRows = 256, kStreamCols = 32;
 inline int stream_tile_cols(int ncols_max) { return ncols_max > 16 ? 32 : 16; }
-hipError_t launch_stream(bool trans, int ncols_max, bool vec, const StreamDesc* d, int n, int64_t tiles,
-                         hipStream_t s, const int32_t* gate = nullptr);
+hipError_t launch_stream(int ncols_max, bool vec, const StreamDesc* d, int n, int64_t tiles, hipStream_t s,
+                         const int32_t* gate = nullptr);
 hipError_t launch_sum(const SumDesc* d, int n, int64_t tiles, hipStream_t s,
                       const int32_t* gate = nullptr);  // double partials
 hipError_t launch_copy(DT ts, DT td, const CopyDesc* d, int n, int64_t tiles, hipStream_t s,

@@ -675,18 +675,20 @@ struct Builder {
   }
 
   // FP64 products with few columns (the residual and matvec sweeps), given as
-  // streaming descriptors: the streaming kernel (A read once, no LDS) while
-  // every product is at most kStreamCols wide (16 with `trans`), else the
-  // tile engine's 64 x 32 LDS tiles, two launches.  Measured at N = 2^20,
-  // 1024-row leaves, 32 columns: the rows product K x + U Z runs at 0.68 of
-  // HBM streaming (vector loads) against 0.58 on the engine, while V^T x
-  // streams at 0.35-0.38 (a wave's 64 columns of V, 8 MB apart, each read in
-  // 128-256 byte pieces) against 0.59 on the engine's LDS tiles
+  // streaming descriptors: A B products at most kStreamCols wide on the
+  // streaming kernel (A read once, no LDS), V^T x products and wider ones on
+  // the tile engine's 64 x 32 LDS tiles (two launches when there is a second
+  // segment).  Measured at N = 2^20, 1024-row leaves, 32 columns: the rows
+  // product K x + U Z runs at 0.68 of HBM streaming (vector loads) against 0.58
+  // on the engine, while V^T x streams at 0.35-0.38 (a wave's 64 columns of V,
+  // 8 MB apart, each read in 128-256 byte pieces) against 0.55-0.59 on the
+  // engine's LDS tiles; at 8 columns (N = 262,144, rank 100) 0.34 streaming
+  // against 0.55 on the engine
   void few_cols(Proc& pr, Cat cat, bool trans, const std::vector<StreamDesc>& v) {
     int nmax = 0;
     for (auto& x : v) nmax = std::max(nmax, x.n);
-    if (nmax <= (trans ? 16 : kStreamCols)) {
-      stream(pr, cat, trans, v);
+    if (!trans && nmax <= kStreamCols) {
+      stream(pr, cat, v);
       return;
     }
     std::vector<G> g1, g2;   // op(A1) B1 (+ beta Cin), then += A2 B2 in a second launch
@@ -706,21 +708,22 @@ struct Builder {
     gemm(pr, cat, F64, F64, F64, F64, false, 4, g2);
   }
 
-  // FP64 streaming products, tiles of kStreamRows rows x stream_tile_cols(nmax) columns
-  void stream(Proc& pr, Cat cat, bool trans, std::vector<StreamDesc> v) {
+  // FP64 streaming products C = alpha (A1 B1 + A2 B2) + beta Cin, tiles of
+  // kStreamRows rows x stream_tile_cols(nmax) columns
+  void stream(Proc& pr, Cat cat, std::vector<StreamDesc> v) {
     int nmax = 0;
     for (auto& x : v) nmax = std::max(nmax, x.n);
     const int nc = stream_tile_cols(nmax);
     int64_t t = 0;
     double flops = 0, bytes = 0;
     std::vector<StreamDesc> d;
-    // 32-byte vector loads need aligned operands (and whole row quads for NN)
+    // 32-byte vector loads need aligned operands and whole row quads
     auto al = [](const void* ptr, int64_t ld) { return (uintptr_t)ptr % 32 == 0 && ld % 4 == 0; };
     bool vec = true;
     for (auto& x : v) {
       if (x.m <= 0 || x.n <= 0 || x.k1 + x.k2 <= 0) continue;
-      if (!al(x.A1, x.lda1) || !al(x.B1, x.ldb1) || (!trans && x.m % 4 != 0) ||
-          (x.k2 > 0 && (!al(x.A2, x.lda2) || !al(x.B2, x.ldb2) || x.m % 4 != 0)))
+      if (!al(x.A1, x.lda1) || !al(x.B1, x.ldb1) || x.m % 4 != 0 ||
+          (x.k2 > 0 && (!al(x.A2, x.lda2) || !al(x.B2, x.ldb2))))
         vec = false;
     }
     for (auto& x : v) {
@@ -736,7 +739,7 @@ struct Builder {
     }
     if (d.empty()) return;
     Launch L;
-    L.kind = K_STREAM; L.cat = cat; L.trans = trans; L.nmax = nmax; L.mode = vec;
+    L.kind = K_STREAM; L.cat = cat; L.nmax = nmax; L.mode = vec;
     L.off = blob.add(d); L.count = int(d.size()); L.tiles = t; L.flops = flops; L.bytes = bytes;
     pr.l.push_back(L);
   }
@@ -1302,7 +1305,7 @@ hipError_t dispatch(hodlr_plan_s* P, const Launch& L) {
     case K_SUM: return launch_sum((const SumDesc*)d, L.count, L.tiles, s, gate);
     case K_COPY: return launch_copy(L.t[0], L.t[1], (const CopyDesc*)d, L.count, L.tiles, s, gate);
     case K_CHECK: return launch_refine_check((const CheckDesc*)d, L.count, ctl, L.mode, s, gate);
-    case K_STREAM: return launch_stream(L.trans, L.nmax, L.mode != 0, (const StreamDesc*)d, L.count, L.tiles, s, gate);
+    case K_STREAM: return launch_stream(L.nmax, L.mode != 0, (const StreamDesc*)d, L.count, L.tiles, s, gate);
     case K_FRAG: return launch_frag_bf16(L.mode != 0, (const FragDesc*)d, L.count, L.tiles, s, gate);
     case K_LU_INIT: return launch_lu_init((const LuDesc*)d, L.count, s);
     case K_LU_PANEL: return launch_lu_panel(L.t[0], (const LuDesc*)d, L.count, L.k0, L.rows_max, s);
